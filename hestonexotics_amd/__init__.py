@@ -1,0 +1,9 @@
+"""hestonexotics_amd -- MI355X-native Heston QE Monte-Carlo pricing (gfx950 HIP behind a C ABI).
+
+The product is libhexo.so (hestonexotics_amd/csrc); this package is its Python binding.
+"""
+from .api import (ASIAN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64, TRAPEZOID, Batch, Context,
+                  HexoError, Request, time_grid, validate)
+
+__all__ = ["ASIAN", "DISCOUNT", "DRIFT", "EUROPEAN", "FP32", "FP32_COMPENSATED", "FP64", "TRAPEZOID", "Batch",
+           "Context", "HexoError", "Request", "time_grid", "validate"]

@@ -1,0 +1,337 @@
+"""Python mirror of the pricing interface, over the C ABI (include/hexo.h).
+
+`Request` carries what one call of the reference's
+`HSimulation::price<HQEAnderson<ffloat, Policy>>(p, S, chains, n_simulations, n_opts, steps)`
+(src/inc/HSimulation.h:61-63) needs: Heston parameters in HParams order, spot, the expiry chains
+with their strikes, path count, steps per expiry, plus the RNG seed, the payoffs to price from the
+same paths, the precision and the documented extensions (drift, discount, corrected trapezoid).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ASIAN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64, TRAPEZOID, HexoError,
+                   hexo_batch_stats, hexo_request, hexo_server_config, hexo_server_stats, hexo_ticket_info,
+                   hexo_tuning)
+
+__all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "DRIFT", "DISCOUNT", "TRAPEZOID", "FP64", "FP32",
+           "FP32_COMPENSATED", "HexoError", "validate", "time_grid"]
+
+
+@dataclass
+class Request:
+    v0: float
+    theta: float
+    rho: float
+    kappa: float
+    sigma: float
+    S0: float
+    expiries: Sequence[float]
+    strikes: Sequence[Sequence[float]]  # one list of strikes per expiry
+    steps: int
+    n_paths: int
+    seed: int = 1
+    r: float = 0.0
+    payoffs: int = ASIAN | EUROPEAN
+    flags: int = 0
+    precision: int = FP32
+    path_offset: int = 0
+    _keep: list = field(default_factory=list, repr=False, compare=False)
+
+    @property
+    def params(self) -> list[float]:
+        return [self.v0, self.theta, self.rho, self.kappa, self.sigma]
+
+    @property
+    def n_opts(self) -> int:
+        return int(sum(len(k) for k in self.strikes))
+
+    @property
+    def n_payoffs(self) -> int:
+        return (1 if self.payoffs & ASIAN else 0) + (1 if self.payoffs & EUROPEAN else 0)
+
+    @property
+    def n_outputs(self) -> int:
+        return self.n_opts * self.n_payoffs
+
+    def flat_strikes(self) -> np.ndarray:
+        return np.ascontiguousarray(np.concatenate([np.asarray(k, np.float64) for k in self.strikes])
+                                    if self.strikes else np.zeros(0), np.float64)
+
+    def n_strikes(self) -> np.ndarray:
+        return np.ascontiguousarray([len(k) for k in self.strikes], np.uint32)
+
+    def to_c(self) -> hexo_request:
+        T = np.ascontiguousarray(self.expiries, np.float64)
+        ns = self.n_strikes()
+        K = self.flat_strikes()
+        if K.size == 0:
+            K = np.zeros(1, np.float64)
+        self._keep = [T, ns, K]
+        q = hexo_request()
+        q.v0, q.theta, q.rho, q.kappa, q.sigma = self.params
+        q.S0, q.r = self.S0, self.r
+        q.n_paths, q.seed, q.path_offset = int(self.n_paths), int(self.seed) & (2**64 - 1), int(self.path_offset)
+        q.steps, q.n_expiries = int(self.steps), len(T)
+        q.expiries = T.ctypes.data_as(C.POINTER(C.c_double))
+        q.n_strikes = ns.ctypes.data_as(C.POINTER(C.c_uint32))
+        q.strikes = K.ctypes.data_as(C.POINTER(C.c_double))
+        q.payoffs, q.flags, q.precision, q.reserved = self.payoffs, self.flags, self.precision, 0
+        return q
+
+    def split(self, prices: np.ndarray) -> dict:
+        """Outputs are payoff-major: the Asian block (if priced) then the European block."""
+        out, o = {}, 0
+        n = self.n_opts
+        if self.payoffs & ASIAN:
+            out["asian"] = prices[o:o + n]
+            o += n
+        if self.payoffs & EUROPEAN:
+            out["european"] = prices[o:o + n]
+        return out
+
+
+def _c_array(reqs: Sequence[Request]):
+    arr = (hexo_request * len(reqs))()
+    for i, r in enumerate(reqs):
+        arr[i] = r.to_c()
+    return arr
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def validate(req: Request) -> int:
+    lib = _lib.load()
+    return lib.hexo_validate_request(C.byref(req.to_c()))
+
+
+def time_grid(expiries, n_strikes, steps: int):
+    lib = _lib.load()
+    T = np.ascontiguousarray(expiries, np.float64)
+    ns = np.ascontiguousarray(n_strikes, np.uint32)
+    fs = np.zeros(len(T), np.uint64)
+    fw = np.zeros(len(T), np.float64)
+    tot = C.c_uint64(0)
+    rc = lib.hexo_time_grid(_dp(T), len(T), ns.ctypes.data_as(C.POINTER(C.c_uint32)), steps,
+                            fs.ctypes.data_as(C.POINTER(C.c_uint64)), _dp(fw), C.byref(tot))
+    if rc != 0:
+        raise HexoError(rc, "time grid")
+    return int(tot.value), fs, fw
+
+
+class Batch:
+    """A set of requests resident on the GPU; its launch sequence is a captured HIP graph."""
+
+    def __init__(self, ctx: "Context", reqs: Sequence[Request]):
+        self.ctx = ctx
+        self.reqs = list(reqs)
+        self._arr = _c_array(self.reqs)
+        self.n_out = sum(r.n_outputs for r in self.reqs)
+        self._h = C.c_void_p()
+        ctx._check(ctx.lib.hexo_batch_create(ctx.h, self._arr, len(self.reqs), C.byref(self._h)))
+
+    def run(self):
+        self.ctx._check(self.ctx.lib.hexo_batch_run(self._h))
+
+    def run_eager(self):
+        self.ctx._check(self.ctx.lib.hexo_batch_run_eager(self._h))
+
+    def sync(self):
+        self.ctx._check(self.ctx.lib.hexo_batch_sync(self._h))
+
+    def results(self):
+        p = np.zeros(self.n_out, np.float64)
+        s = np.zeros(self.n_out, np.float64)
+        self.ctx._check(self.ctx.lib.hexo_batch_results(self._h, _dp(p), _dp(s)))
+        return p, s
+
+    def per_request(self, arr: np.ndarray) -> list[np.ndarray]:
+        out, o = [], 0
+        for r in self.reqs:
+            out.append(arr[o:o + r.n_outputs])
+            o += r.n_outputs
+        return out
+
+    def stats(self) -> dict:
+        st = hexo_batch_stats()
+        self.ctx._check(self.ctx.lib.hexo_batch_stats_get(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def branch_counts(self) -> np.ndarray:
+        c = np.zeros(4, np.uint64)
+        self.ctx._check(self.ctx.lib.hexo_batch_branch_counts(self._h, c.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return c
+
+    def close(self):
+        if self._h:
+            self.ctx.lib.hexo_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One GPU. Wraps hexo_ctx_create / hexo_ctx_destroy."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        self.h = C.c_void_p()
+        rc = self.lib.hexo_ctx_create(device, C.byref(self.h))
+        if rc != 0:
+            raise HexoError(rc, "hexo_ctx_create")
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise HexoError(rc, self.lib.hexo_ctx_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.hexo_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def device_info(self) -> dict:
+        name = C.create_string_buffer(128)
+        cu, clk = C.c_int(0), C.c_int(0)
+        self._check(self.lib.hexo_device_info(self.h, name, 128, C.byref(cu), C.byref(clk)))
+        return {"arch": name.value.decode(), "n_cu": cu.value, "clock_khz": clk.value}
+
+    # ---- pricing ----
+    def price(self, req: Request):
+        p = np.zeros(req.n_outputs, np.float64)
+        s = np.zeros(req.n_outputs, np.float64)
+        self._check(self.lib.hexo_price(self.h, C.byref(req.to_c()), _dp(p), _dp(s)))
+        return p, s
+
+    def price_batch(self, reqs: Sequence[Request]):
+        arr = _c_array(reqs)
+        n = sum(r.n_outputs for r in reqs)
+        p = np.zeros(n, np.float64)
+        s = np.zeros(n, np.float64)
+        self._check(self.lib.hexo_price_batch(self.h, arr, len(reqs), _dp(p), _dp(s)))
+        out, o = [], 0
+        for r in reqs:
+            out.append((p[o:o + r.n_outputs], s[o:o + r.n_outputs]))
+            o += r.n_outputs
+        return out
+
+    def batch(self, reqs: Sequence[Request]) -> Batch:
+        return Batch(self, reqs)
+
+    def paths(self, req: Request, first: int, n: int) -> np.ndarray:
+        out = np.zeros((n, len(req.expiries), 2), np.float64)
+        self._check(self.lib.hexo_paths(self.h, C.byref(req.to_c()), first, n, _dp(out.reshape(-1))))
+        return out
+
+    # ---- components ----
+    def rng_fill(self, seed: int, path0: int, n_paths: int, word0: int, n_words: int, stream: int = 0):
+        out = np.zeros((n_paths, n_words), np.uint32)
+        self._check(self.lib.hexo_rng_fill(self.h, seed, stream, path0, n_paths, word0, n_words,
+                                           out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
+
+    def philox(self, ctr: np.ndarray, key: np.ndarray) -> np.ndarray:
+        ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 4)
+        key = np.ascontiguousarray(key, np.uint32).reshape(-1, 2)
+        out = np.zeros_like(ctr)
+        u = C.POINTER(C.c_uint32)
+        self._check(self.lib.hexo_philox4x32_10(self.h, ctr.ctypes.data_as(u), key.ctypes.data_as(u), len(ctr),
+                                                out.ctypes.data_as(u)))
+        return out
+
+    def normals(self, words: np.ndarray, precision: int) -> np.ndarray:
+        w = np.ascontiguousarray(words, np.uint32).reshape(-1)
+        out = np.zeros(w.size, np.float64)
+        self._check(self.lib.hexo_normals(self.h, w.ctypes.data_as(C.POINTER(C.c_uint32)), w.size, precision,
+                                          _dp(out)))
+        return out
+
+    # ---- tuning / measurement ----
+    def get_tuning(self) -> dict:
+        t = hexo_tuning()
+        self._check(self.lib.hexo_get_tuning(self.h, C.byref(t)))
+        return {"paths_per_thread": t.paths_per_thread, "block_threads": t.block_threads,
+                "waves_per_simd": t.waves_per_simd}
+
+    def set_tuning(self, paths_per_thread: int, block_threads: int, waves_per_simd: int = 0):
+        t = hexo_tuning(paths_per_thread, block_threads, waves_per_simd, 0)
+        self._check(self.lib.hexo_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, reqs: Sequence[Request], reps: int = 2) -> dict:
+        arr = _c_array(reqs)
+        t = hexo_tuning()
+        self._check(self.lib.hexo_autotune(self.h, arr, len(reqs), reps, C.byref(t)))
+        return {"paths_per_thread": t.paths_per_thread, "block_threads": t.block_threads,
+                "waves_per_simd": t.waves_per_simd}
+
+    def copy_probe(self, nbytes: int, reps: int = 10) -> float:
+        ms = C.c_double(0)
+        self._check(self.lib.hexo_copy_probe(self.h, nbytes, reps, C.byref(ms)))
+        return ms.value
+
+    # ---- serving ----
+    def server_start(self, memory_budget_bytes: int = 0, streams: int = 2, max_batch: int = 0):
+        cfg = hexo_server_config(memory_budget_bytes, streams, max_batch, 0.0)
+        self._check(self.lib.hexo_server_start(self.h, C.byref(cfg)))
+
+    def server_stop(self):
+        self._check(self.lib.hexo_server_stop(self.h))
+
+    def submit(self, req: Request) -> int:
+        t = C.c_uint64(0)
+        rc = self.lib.hexo_submit(self.h, C.byref(req.to_c()), C.byref(t))
+        if rc != 0:
+            raise HexoError(rc, self.lib.hexo_ctx_last_error(self.h).decode())
+        return t.value
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int(0)
+        self._check(self.lib.hexo_poll(self.h, ticket, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket: int, n_out: int, timeout_s: float = -1.0):
+        p = np.zeros(n_out, np.float64)
+        s = np.zeros(n_out, np.float64)
+        info = hexo_ticket_info()
+        rc = self.lib.hexo_wait(self.h, ticket, timeout_s, _dp(p), _dp(s), C.byref(info))
+        if rc != 0:
+            raise HexoError(rc, "wait")
+        return p, s, {k: getattr(info, k) for k, _ in info._fields_}
+
+    def cancel(self, ticket: int):
+        self._check(self.lib.hexo_cancel(self.h, ticket))
+
+    def server_stats(self) -> dict:
+        st = hexo_server_stats()
+        self._check(self.lib.hexo_server_stats_get(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def inject_fault(self, kind: int, count: int):
+        self._check(self.lib.hexo_inject_fault(self.h, kind, count))
+
+    def run_trace(self, reqs: Sequence[Request], arrival_s: Sequence[float]):
+        arr = _c_array(reqs)
+        n = sum(r.n_outputs for r in reqs)
+        p = np.zeros(max(n, 1), np.float64)
+        s = np.zeros(max(n, 1), np.float64)
+        info = (hexo_ticket_info * max(len(reqs), 1))()
+        at = np.ascontiguousarray(arrival_s, np.float64)
+        rc = self.lib.hexo_server_run_trace(self.h, arr, _dp(at), len(reqs), _dp(p), _dp(s), info)
+        infos = [{k: getattr(info[i], k) for k, _ in info[i]._fields_} for i in range(len(reqs))]
+        return rc, p[:n], s[:n], infos

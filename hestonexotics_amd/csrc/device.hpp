@@ -1,0 +1,148 @@
+// device.hpp -- device building blocks of the Heston QE path kernels (gfx950).
+//
+//  * Philox4x32-10 counter-based generator (Salmon et al., SC'11). Replaces the reference's
+//    shishua ring buffers (src/inc/RNG.h:17-52, src/RNG.cpp:8-47): any (seed, path, step) draw is
+//    computed directly, so paths can be split across waves, launches or GPUs freely.
+//  * Uniforms u = (k + 1/2) 2^-32 from a 32-bit word k (the reference maps a 64-bit word,
+//    src/RNG.cpp:31). u is never 0 or 1, and 1 - u(k) = u(~k) exactly.
+//  * AS241 inverse normal (Wichura 1988; src/as241.f90:15-119): an FP64 version in the
+//    reference's form and an FP32 version that forms q = u - 1/2 and min(u, 1-u) from the
+//    integer, evaluates the central and tail rationals without branching and selects.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hexo {
+
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+
+// One counter block: ctr = {c0, c1, c2, c3}, key = {k0, k1}. When c0, c2, c3 and the key are
+// wave-uniform (step index, path high word, stream, seed) the first round runs on the scalar unit.
+__device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                               uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = (uint32_t)p1;
+        c2 = n2;
+        c3 = (uint32_t)p0;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    return u32x4{c0, c1, c2, c3};
+}
+
+// ---- AS241 coefficients (src/as241.f90:31-44 A/B, :49-63 C/D, :68-82 E/F) ----
+#define HEXO_AS241_A 3.3871328727963666080e0, 1.3314166789178437745e+2, 1.9715909503065514427e+3, \
+    1.3731693765509461125e+4, 4.5921953931549871457e+4, 6.7265770927008700853e+4, 3.3430575583588128105e+4, \
+    2.5090809287301226727e+3
+#define HEXO_AS241_B 1.0, 4.2313330701600911252e+1, 6.8718700749205790830e+2, 5.3941960214247511077e+3, \
+    2.1213794301586595867e+4, 3.9307895800092710610e+4, 2.8729085735721942674e+4, 5.2264952788528545610e+3
+#define HEXO_AS241_C 1.42343711074968357734e0, 4.63033784615654529590e0, 5.76949722146069140550e0, \
+    3.64784832476320460504e0, 1.27045825245236838258e0, 2.41780725177450611770e-1, 2.27238449892691845833e-2, \
+    7.74545014278341407640e-4
+#define HEXO_AS241_D 1.0, 2.05319162663775882187e0, 1.67638483018380384940e0, 6.89767334985100004550e-1, \
+    1.48103976427480074590e-1, 1.51986665636164571966e-2, 5.47593808499534494600e-4, 1.05075007164441684324e-9
+#define HEXO_AS241_E 6.65790464350110377720e0, 5.46378491116411436990e0, 1.78482653991729133580e0, \
+    2.96560571828504891230e-1, 2.65321895265761230930e-2, 1.24266094738807843860e-3, 2.71155556874348757815e-5, \
+    2.01033439929228813265e-7
+#define HEXO_AS241_F 1.0, 5.99832206555887937690e-1, 1.36929880922735805310e-1, 1.48753612908506148525e-2, \
+    7.86869131145613259100e-4, 1.84631831751005468180e-5, 1.42151175831644588870e-7, 2.04426310338993978564e-15
+
+template <typename T>
+struct As241 {
+    static constexpr T a[8] = {HEXO_AS241_A};
+    static constexpr T b[8] = {HEXO_AS241_B};
+    static constexpr T c[8] = {HEXO_AS241_C};
+    static constexpr T d[8] = {HEXO_AS241_D};
+    static constexpr T e[8] = {HEXO_AS241_E};
+    static constexpr T f[8] = {HEXO_AS241_F};
+};
+
+// ((c7 r + c6) r + ...) r + c0 with fused multiply-adds; a leading coefficient of 1 (B, D, F
+// have c0 = 1) is handled by the same chain.
+template <typename T>
+__device__ __forceinline__ T horner8(const T (&c)[8], T r) {
+    T acc = c[7];
+#pragma unroll
+    for (int i = 6; i >= 0; --i) acc = __builtin_fma(acc, r, c[i]);
+    return acc;
+}
+__device__ __forceinline__ float horner8f(const float (&c)[8], float r) {
+    float acc = c[7];
+#pragma unroll
+    for (int i = 6; i >= 0; --i) acc = __builtin_fmaf(acc, r, c[i]);
+    return acc;
+}
+
+__device__ __forceinline__ double uniform_f64(uint32_t k) { return ((double)k + 0.5) * 0x1p-32; }
+// (k + 1/2) 2^-32 and 1 - that, in FP32, from the integer.
+__device__ __forceinline__ float uniform_f32(uint32_t k) { return __builtin_fmaf((float)k, 0x1p-32f, 0x1p-33f); }
+__device__ __forceinline__ float one_minus_uniform_f32(uint32_t k) { return uniform_f32(~k); }
+
+// AS241 in FP64, the reference's region logic (src/as241.f90:85-118) on u = (k + 1/2) 2^-32.
+// u never reaches 0 or 1, so the IFAULT branch cannot trigger; 32-bit inputs never reach the
+// far tail (r <= 4.78 < 5), which is kept for completeness.
+__device__ __forceinline__ double normal_f64(uint32_t k) {
+    const double p = uniform_f64(k);
+    const double q = p - 0.5;
+    if (__builtin_fabs(q) <= 0.425) {
+        const double r = 0.180625 - q * q;
+        return q * horner8(As241<double>::a, r) / horner8(As241<double>::b, r);
+    }
+    double r = (q < 0.0) ? p : 1.0 - p;
+    r = ::sqrt(-::log(r));
+    double z;
+    if (r <= 5.0) {
+        r -= 1.6;
+        z = horner8(As241<double>::c, r) / horner8(As241<double>::d, r);
+    } else {
+        r -= 5.0;
+        z = horner8(As241<double>::e, r) / horner8(As241<double>::f, r);
+    }
+    return (q < 0.0) ? -z : z;
+}
+
+// AS241 in FP32, driven by the integer. q = (k - 2^31 + 1/2) 2^-32 and the tail argument
+// min(u, 1-u) = (min(k, ~k) + 1/2) 2^-32 keep full relative precision in both tails. Both
+// rational approximations are evaluated and the region is selected: with 15% tail draws a
+// 64-lane wave almost always needs both anyway, and the select avoids the branch and the
+// exec-mask bookkeeping. The far-tail rational (r > 5) is unreachable from 32-bit inputs.
+__device__ __forceinline__ float normal_f32(uint32_t k) {
+    const int32_t sk = (int32_t)(k ^ 0x80000000u);
+    const float q = __builtin_fmaf((float)sk, 0x1p-32f, 0x1p-33f);
+    // central region |q| <= 0.425
+    const float rc = __builtin_fmaf(-q, q, 0.180625f);
+    const float nc = q * horner8f(As241<float>::a, rc);
+    const float dc = horner8f(As241<float>::b, rc);
+    // tail: r = sqrt(-ln(min(u,1-u))) - 1.6
+    const uint32_t mn = k ^ ~(uint32_t)(sk >> 31);
+    const float rr = __builtin_fmaf((float)mn, 0x1p-32f, 0x1p-33f);
+    const float rt = __builtin_amdgcn_sqrtf(-0.693147180559945309f * __builtin_amdgcn_logf(rr)) - 1.6f;
+    float nt = horner8f(As241<float>::c, rt);
+    const float dt = horner8f(As241<float>::d, rt);
+    nt = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, nt) ^ ((uint32_t)sk & 0x80000000u));
+    const bool central = __builtin_fabsf(q) <= 0.425f;
+    const float num = central ? nc : nt;
+    const float den = central ? dc : dt;
+    return num * __builtin_amdgcn_rcpf(den);
+}
+
+__device__ __forceinline__ bool normal_is_central(uint32_t k) {
+    const int32_t sk = (int32_t)(k ^ 0x80000000u);
+    const float q = __builtin_fmaf((float)sk, 0x1p-32f, 0x1p-33f);
+    return __builtin_fabsf(q) <= 0.425f;
+}
+
+}  // namespace hexo

@@ -1,0 +1,139 @@
+// engine.hpp -- host engine of the Heston QE pricer: contexts, schedules, batches.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/hexo.h"
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace hexo {
+
+// ---- the reference's time grid, as runs of equal steps ------------------------------------
+struct HostFin {
+    uint32_t chain;
+    double T, tp, dt, w;
+};
+struct HostSeg {
+    uint32_t n_steps = 0;
+    double delta = 0.0;
+    double dnext = 0.0;
+    std::vector<HostFin> fins;
+};
+// Replays src/HSimulation.tpp:30-45 (with src/inc/AsianContract.h:35-38) in double and cuts the
+// step sequence at every expiry finalisation. Returns false on bad input.
+bool build_schedule(const double* T, uint32_t n_exp, const uint32_t* n_strikes, uint32_t steps,
+                    std::vector<HostSeg>& segs, uint64_t& total_steps);
+
+std::string validate(const hexo_request* r);  // "" if valid, else the reason
+uint32_t n_payoffs(uint32_t payoffs);
+uint64_t n_opts_of(const hexo_request* r);
+
+// ---- device memory ------------------------------------------------------------------------
+// First-fit sub-allocator over one device allocation (serving memory budget).
+class Arena {
+   public:
+    bool init(size_t bytes);
+    void release();
+    void* alloc(size_t bytes);
+    void free(void* p);
+    size_t capacity() const { return cap_; }
+    size_t in_use() const { return used_; }
+    ~Arena() { release(); }
+
+   private:
+    char* base_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+    std::vector<std::pair<size_t, size_t>> free_;  // (offset, size), sorted by offset
+    std::unordered_map<size_t, size_t> live_;      // offset -> size
+};
+
+// ---- a batch of requests resident on the device --------------------------------------------
+struct Batch {
+    hexo_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t n_req = 0;
+    std::vector<uint64_t> out_base;  // per request, into the concatenated outputs
+    uint64_t n_out_total = 0;
+    double path_steps = 0.0, nominal_path_steps = 0.0;
+    // device memory (one region)
+    char* d_mem = nullptr;
+    size_t d_bytes = 0;
+    Arena* arena = nullptr;  // owner of d_mem when not hipMalloc'ed
+    // host pinned staging
+    char* h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+    size_t upload_bytes = 0;
+    double* h_prices = nullptr;
+    double* h_ses = nullptr;
+    // kernels: class 0 = FP64 kernel, class 1 = FP32 kernel
+    PathArgs pa[2]{};
+    uint32_t grid[2]{}, block[2]{}, ppt[2]{};
+    bool has_class[2]{};
+    ReduceArgs ra{};
+    uint64_t n_group_threads = 0, n_out_threads = 0;
+    uint32_t* d_counters = nullptr;
+    unsigned long long* d_counts = nullptr;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // timing
+    hipEvent_t ev[3]{};
+    double last_run_ms = 0, path_ms = 0, reduce_ms = 0;
+    uint64_t path_launches = 0;
+    std::chrono::steady_clock::time_point t_run;
+    bool timed = false;
+    ~Batch();
+};
+
+}  // namespace hexo
+
+struct hexo_ctx {
+    int device = 0;
+    hipDeviceProp_t prop{};
+    hipStream_t stream = nullptr;
+    hexo_tuning tuning{2, 256, 0, 0};
+    std::recursive_mutex mu;
+    std::string last_error;
+    struct hexo_server_state* server = nullptr;  // owned; see server.cpp
+    int set_error(int code, const std::string& msg) {
+        last_error = msg;
+        return code;
+    }
+};
+
+struct hexo_batch {
+    std::unique_ptr<hexo::Batch> b;
+};
+
+namespace hexo {
+
+// Creates a batch: validates, builds the schedules, packs and uploads the tables, plans the
+// launches. mem/arena: when arena is non-null the device region comes from it.
+int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_t stream, Arena* arena,
+                 std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n = nullptr);
+// Device bytes a set of requests needs (admission accounting).
+size_t batch_device_bytes(const hexo_request* reqs, uint32_t n);
+size_t request_device_bytes(const hexo_request* r);
+// Enqueue the batch's launches (eager) on its stream; timed = record HIP events around them.
+int batch_enqueue(Batch& b, PathMode mode, bool timed, double* d_finals, std::string& err);
+int batch_capture_graph(Batch& b, std::string& err);
+int batch_launch_graph(Batch& b, std::string& err);
+int batch_finish(Batch& b, std::string& err);  // synchronise + timing readout
+void batch_copy_results(const Batch& b, double* prices, double* ses);
+void batch_destroy(Batch& b);
+
+double now_s();
+
+}  // namespace hexo

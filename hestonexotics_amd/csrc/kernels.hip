@@ -1,0 +1,710 @@
+// kernels.hip -- HIP kernels of the Heston QE Monte-Carlo path for gfx950.
+//
+// k_path_fp32  fused FP32 path kernel: Philox4x32-10 -> AS241 -> QE step -> Asian/European
+//              finalisation -> per-tile payoff sums. Replaces the reference's OpenMP path loop
+//              and scheme (src/HSimulation.tpp:23-49, :52-86), the option policies
+//              (src/inc/AsianContract.h:25-47, src/inc/VanillaContract.h:24-39) and the RNG ring
+//              refills (src/RNG.cpp:28-43, src/as241.f90:85-118).
+// k_path_fp64  the plain FP64 kernel: the same path in the reference's own FP64 form.
+// k_reduce_*   cross-tile reduction in a fixed order and the estimator (price, standard error);
+//              replaces the racy `prices[...] +=` of src/HSimulation.tpp:40,47-48.
+//
+// Work distribution: a persistent grid of independent wavefronts. Each wave takes a work item
+// (request, PPT consecutive 64-path tiles) from an atomic queue; items are ordered longest request
+// first. Lane l of the wave runs path tile*64 + l of each of its PPT tiles. At every expiry the
+// wave transposes its paths' (A, X_T) through LDS so that lanes work over (option, 8-path group)
+// pairs; each tile's sums therefore depend only on its 64 paths, never on the launch shape.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace hexo {
+
+namespace {
+
+constexpr uint32_t kAsian = 1u;
+constexpr uint32_t kTrapezoid = 4u;
+constexpr uint32_t kPrecFP64 = 0u;
+constexpr uint32_t kPrecComp = 2u;
+
+__device__ __forceinline__ void wave_sync_lds() {
+    // LDS traffic of one wave is processed in order; this keeps the compiler from moving the
+    // transpose's reads above its writes.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Work item -> (sorted request position). item_end is a prefix sum over the sorted requests.
+__device__ __forceinline__ uint32_t find_sorted(const uint32_t* item_end, uint32_t n, uint32_t item) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (item_end[mid] > item)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t next_item(uint32_t* counter, int lane) {
+    uint32_t item = 0;
+    if (lane == 0) item = atomicAdd(counter, 1u);
+    item = __shfl(item, 0);
+    return uniform(item);
+}
+
+// Compensated (Neumaier) FP32 accumulation.
+struct CompF {
+    float s, c;
+    __device__ __forceinline__ void add(float x) {
+        const float t = s + x;
+        c += (__builtin_fabsf(s) >= __builtin_fabsf(x)) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ void merge(float s2, float c2) {
+        const float t = s + s2;
+        const float e = (__builtin_fabsf(s) >= __builtin_fabsf(s2)) ? (s - t) + s2 : (s2 - t) + s;
+        s = t;
+        c = c + c2 + e;
+    }
+};
+
+// Per-expiry tile reduction. vals[j*64 + p] holds (A, X_T) of path p of sub-tile j (in the
+// kernel's units). Lane (o = lane/8, g = lane%8) sums the payoffs of paths 8g..8g+7 for output o
+// in path order; the 8 group sums are combined by a fixed xor tree. Output o of the request is
+// payoff-major: o = pay * n_opts + option.
+template <int PPT, typename ValT>
+__device__ __forceinline__ void reduce_tiles(const PathArgs& a, const DevReq& R, const DevFin& F, uint32_t tile0,
+                                             int lane, const ValT* vals, bool fp32_units) {
+    const uint32_t nk = F.n_strikes;
+    const uint32_t npay = R.n_pay;
+    const uint32_t n_out_c = nk * npay;
+    const uint32_t n_out = R.n_opts * npay;
+    const int g = lane & 7;
+    const bool comp = R.precision == kPrecComp;
+    for (uint32_t base = 0; base < n_out_c; base += 8) {
+        const uint32_t ol = base + (uint32_t)(lane >> 3);
+        const bool active = ol < n_out_c;
+        const uint32_t olc = active ? ol : 0u;
+        const uint32_t pay = olc / nk;
+        const uint32_t kj = olc - pay * nk;
+        const bool use_xt = (pay == 1u) || !(R.payoffs & kAsian);
+        const DevOpt& opt = a.opts[R.opt_begin + F.opt_begin + kj];
+        const uint32_t o = pay * R.n_opts + F.opt_begin + kj;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const uint32_t tile = tile0 + (uint32_t)j;
+            if (tile >= R.n_tiles) break;
+            const uint64_t p0 = (uint64_t)tile * kTile;
+            const uint64_t n_valid64 = R.n_paths - p0;
+            const int n_valid = n_valid64 >= (uint64_t)kTile ? kTile : (int)n_valid64;
+            const size_t slot = (size_t)R.partial_base + (size_t)tile * n_out + o;
+            if constexpr (sizeof(ValT) == 8) {
+                // FP32 kernel: payoffs in FP32, sums in FP64 or compensated FP32
+                const float K = opt.k_rel;
+                if (!comp) {
+                    double s = 0.0, ss = 0.0;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const int p = g * 8 + i;
+                        const float2 v = reinterpret_cast<const float2*>(vals)[j * kTile + p];
+                        const float x = __builtin_fmaxf((use_xt ? v.y : v.x) - K, 0.0f);
+                        const double xd = p < n_valid ? (double)x : 0.0;
+                        s += xd;
+                        ss = __builtin_fma(xd, xd, ss);
+                    }
+#pragma unroll
+                    for (int m = 1; m < 8; m <<= 1) {
+                        s += __shfl_xor(s, m);
+                        ss += __shfl_xor(ss, m);
+                    }
+                    if (g == 0 && active) reinterpret_cast<double2*>(a.partials)[slot] = make_double2(s, ss);
+                } else {
+                    CompF s{0.f, 0.f}, ss{0.f, 0.f};
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const int p = g * 8 + i;
+                        const float2 v = reinterpret_cast<const float2*>(vals)[j * kTile + p];
+                        float x = __builtin_fmaxf((use_xt ? v.y : v.x) - K, 0.0f);
+                        x = p < n_valid ? x : 0.0f;
+                        s.add(x);
+                        ss.add(x * x);
+                    }
+#pragma unroll
+                    for (int m = 1; m < 8; m <<= 1) {
+                        const float s2 = __shfl_xor(s.s, m), c2 = __shfl_xor(s.c, m);
+                        const float q2 = __shfl_xor(ss.s, m), d2 = __shfl_xor(ss.c, m);
+                        s.merge(s2, c2);
+                        ss.merge(q2, d2);
+                    }
+                    if (g == 0 && active) reinterpret_cast<float4*>(a.partials)[slot] = make_float4(s.s, s.c, ss.s, ss.c);
+                }
+            } else {
+                // FP64 kernel
+                const double K = opt.k_abs;
+                double s = 0.0, ss = 0.0;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int p = g * 8 + i;
+                    const double2 v = reinterpret_cast<const double2*>(vals)[j * kTile + p];
+                    double x = (use_xt ? v.y : v.x) - K;
+                    x = x > 0.0 ? x : 0.0;
+                    const double xd = p < n_valid ? x : 0.0;
+                    s += xd;
+                    ss += xd * xd;
+                }
+#pragma unroll
+                for (int m = 1; m < 8; m <<= 1) {
+                    s += __shfl_xor(s, m);
+                    ss += __shfl_xor(ss, m);
+                }
+                if (g == 0 && active) reinterpret_cast<double2*>(a.partials)[slot] = make_double2(s, ss);
+            }
+        }
+    }
+    (void)fp32_units;
+}
+
+__device__ __forceinline__ void count_flush(unsigned long long* counts, uint32_t c0, uint32_t c1, uint32_t c2,
+                                            uint32_t c3, int lane) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        c0 += __shfl_xor(c0, m);
+        c1 += __shfl_xor(c1, m);
+        c2 += __shfl_xor(c2, m);
+        c3 += __shfl_xor(c3, m);
+    }
+    if (lane == 0) {
+        atomicAdd(&counts[0], (unsigned long long)c0);
+        atomicAdd(&counts[1], (unsigned long long)c1);
+        atomicAdd(&counts[2], (unsigned long long)c2);
+        atomicAdd(&counts[3], (unsigned long long)c3);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// FP32 path kernel
+// ------------------------------------------------------------------------------------------
+struct SegF {
+    float d, mth, c1, c2, k0, kc, kd, k3;
+};
+
+// One QE step in FP32 (Andersen's QE scheme as in src/HSimulation.tpp:52-86): variance by the
+// quadratic branch a (b + Z_V)^2 when psi = s2/m^2 < 1.5, else the exponential branch; then the
+// central log-price step. Y is log2(X/S0); X = 2^Y.
+__device__ __forceinline__ bool qe_step_fp32(float& V, float& Y, float& X, float& Xp, uint32_t wv, uint32_t wx,
+                                             const SegF& c) {
+    const float m = __builtin_fmaf(V, c.d, c.mth);
+    const float s2 = __builtin_fmaf(V, c.c1, c.c2);
+    const float m2 = m * m;
+    const bool quad = s2 < 1.5f * m2;
+    float Vn;
+    if (quad) {
+        const float inv = __builtin_amdgcn_rcpf(s2);
+        const float t = (m2 + m2) * inv;  // 2 / psi
+        const float tm1 = t - 1.0f;
+        const float b2 = tm1 + __builtin_amdgcn_sqrtf(t * tm1);
+        const float b = __builtin_amdgcn_sqrtf(b2);
+        const float aq = m * __builtin_amdgcn_rcpf(1.0f + b2);
+        const float bz = b + normal_f32(wv);
+        Vn = aq * bz * bz;
+    } else {
+        // p = (psi-1)/(psi+1), beta = 2/(m (psi+1)); V' = ln((1-p)/(1-U))/beta if U > p, else 0.
+        // With sm = s2 + m^2: 1-p = 2m^2/sm and 1/beta = sm/(2m).
+        const float sm = s2 + m2;
+        const float lhs = one_minus_uniform_f32(wv) * sm;  // (1-U) sm
+        const float tm2 = m2 + m2;
+        const float lg = __builtin_amdgcn_logf(tm2 * __builtin_amdgcn_rcpf(lhs));
+        const float v = (sm * __builtin_amdgcn_rcpf(m)) * (0.5f * 0.693147180559945309f) * lg;
+        Vn = lhs < tm2 ? v : 0.0f;
+    }
+    const float sum = V + Vn;
+    const float dif = Vn - V;
+    const float inc = __builtin_fmaf(c.kd, dif, __builtin_fmaf(c.kc, sum, c.k0));
+    const float sq = __builtin_amdgcn_sqrtf(c.k3 * sum);
+    Y = __builtin_fmaf(sq, normal_f32(wx), Y + inc);
+    Xp = X;
+    X = __builtin_amdgcn_exp2f(Y);
+    V = Vn;
+    return quad;
+}
+
+template <int PPT, bool DEBUG, bool COUNT>
+__device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32_t tile0, int lane, float2* vals) {
+    const DevReq& R = a.reqs[ri];
+    const uint64_t n_paths = R.n_paths;
+    const uint32_t key0 = R.seed_lo, key1 = R.seed_hi;
+    float V[PPT], Y[PPT], X[PPT], Xp[PPT], P[PPT], acc[PPT], Xs[PPT];
+    uint32_t plo[PPT], phi[PPT], wb0[PPT], wb1[PPT];
+    bool valid[PPT];
+    uint32_t cq = 0, ce = 0, cc = 0, ct = 0;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const uint64_t pl = (uint64_t)(tile0 + (uint32_t)j) * kTile + (uint64_t)lane;
+        valid[j] = pl < n_paths;
+        const uint64_t gp = R.path_offset + pl;
+        plo[j] = (uint32_t)gp;
+        phi[j] = (uint32_t)(gp >> 32);
+        V[j] = R.v0_f;
+        Y[j] = 0.0f;
+        X[j] = 1.0f;
+        Xp[j] = 1.0f;
+        P[j] = 0.0f;
+        acc[j] = 0.0f;
+        Xs[j] = 1.0f;
+        wb0[j] = wb1[j] = 0u;
+    }
+    uint32_t k = 0;
+    const uint32_t n_segs = R.n_segs;
+    for (uint32_t s = 0; s < n_segs; ++s) {
+        const DevSeg& S = a.segs[R.seg_begin + s];
+        const SegF c{S.d, S.mth, S.c1, S.c2, S.k0, S.kc, S.kd, S.k3};
+        const uint32_t n = S.n_steps;
+        for (uint32_t i = 0; i < n; ++i, ++k) {
+            uint32_t wv[PPT], wx[PPT];
+            if ((k & 1u) == 0u) {
+#pragma unroll
+                for (int j = 0; j < PPT; ++j) {
+                    const u32x4 w = philox4x32_10(k >> 1, plo[j], phi[j], 0u, key0, key1);
+                    wv[j] = w.x;
+                    wx[j] = w.y;
+                    wb0[j] = w.z;
+                    wb1[j] = w.w;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < PPT; ++j) {
+                    wv[j] = wb0[j];
+                    wx[j] = wb1[j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) {
+                const bool quad = qe_step_fp32(V[j], Y[j], X[j], Xp[j], wv[j], wx[j], c);
+                if constexpr (COUNT) {
+                    if (valid[j]) {
+                        cq += quad ? 1u : 0u;
+                        ce += quad ? 0u : 1u;
+                        const bool cv = normal_is_central(wv[j]), cx = normal_is_central(wx[j]);
+                        cc += (quad && cv ? 1u : 0u) + (cx ? 1u : 0u);
+                        ct += (quad && !cv ? 1u : 0u) + (cx ? 0u : 1u);
+                    }
+                }
+            }
+            if (i + 1u < n) {
+#pragma unroll
+                for (int j = 0; j < PPT; ++j) P[j] += X[j];
+            }
+        }
+        // Asian accumulation of the run's interior steps: delta * (X_{b-1}/2 + X_b + ... + X_{e-1}/2)
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) acc[j] = __builtin_fmaf(S.delta_f, P[j] + 0.5f * (Xs[j] - Xp[j]), acc[j]);
+        for (uint32_t f = 0; f < S.fin_count; ++f) {
+            const DevFin& F = a.fins[S.fin_begin + f];
+            const float w = F.w_f, invT = F.invT_f;
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) {
+                const float dX = X[j] - Xp[j];
+                const float xt = __builtin_fmaf(dX, w, Xp[j]);
+                const float A = (R.flags & kTrapezoid) ? (acc[j] + F.hwdt_f * (Xp[j] + xt)) * invT
+                                                       : __builtin_fmaf(dX, w, acc[j]) * invT;
+                vals[j * kTile + lane] = make_float2(A, xt);
+                if constexpr (DEBUG) {
+                    if (valid[j] && a.finals) {
+                        const uint64_t pl = (uint64_t)(tile0 + (uint32_t)j) * kTile + (uint64_t)lane;
+                        double* out = a.finals + (pl * R.n_exp + F.chain) * 2;
+                        out[0] = (double)A * R.S0;
+                        out[1] = (double)xt * R.S0;
+                    }
+                }
+            }
+            wave_sync_lds();
+            reduce_tiles<PPT>(a, R, F, tile0, lane, vals, true);
+            wave_sync_lds();
+        }
+        const float hn = 0.5f * S.dnext_f;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            acc[j] = __builtin_fmaf(hn, X[j] + Xp[j], acc[j]);
+            Xs[j] = X[j];
+            P[j] = 0.0f;
+        }
+    }
+    if constexpr (COUNT) count_flush(a.counts, cq, ce, cc, ct, lane);
+}
+
+template <int PPT, bool DEBUG, bool COUNT>
+__global__ __launch_bounds__(256) void k_path_fp32(PathArgs a) {
+    extern __shared__ float2 lds_f2[];
+    const int lane = (int)(threadIdx.x & 63u);
+    float2* vals = lds_f2 + (threadIdx.x >> 6) * (PPT * kTile);
+    for (;;) {
+        const uint32_t item = next_item(a.counter, lane);
+        if (item >= a.total_items) break;
+        const uint32_t sp = uniform(find_sorted(a.item_end, a.n_sorted, item));
+        const uint32_t first = sp ? a.item_end[sp - 1] : 0u;
+        const uint32_t ri = uniform(a.order[sp]);
+        path_fp32<PPT, DEBUG, COUNT>(a, ri, uniform((item - first) * (uint32_t)PPT), lane, vals);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Plain FP64 path kernel: the reference's step and policies in FP64, recomputing the step's
+// moments and K0..K4 every step exactly as written in src/HSimulation.tpp:52-86.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool qe_step_fp64(const DevReq& R, double delta, double& V, double& logX, uint32_t wv,
+                                             uint32_t wx) {
+    const double theta = R.theta, rho = R.rho, kappa = R.kappa, eps = R.sigma;
+    const double e = ::exp(-kappa * delta);
+    const double m = theta + (V - theta) * e;
+    const double s2 = __builtin_fabs(V * eps * eps * e / kappa * (1.0 - e) +
+                                     theta * eps * eps / (2.0 * kappa) * (1.0 - e) * (1.0 - e));
+    const double psi = s2 / (m * m);
+    const double Vp = V;
+    const bool quad = psi < 1.5;
+    if (quad) {
+        const double bp2 = 2.0 / psi - 1.0 + ::sqrt(2.0 / psi * (2.0 / psi - 1.0));
+        const double b = ::sqrt(bp2);
+        const double aq = m / (1.0 + bp2);
+        const double zv = normal_f64(wv);
+        V = aq * (b + zv) * (b + zv);
+    } else {
+        const double p = (psi - 1.0) / (psi + 1.0);
+        const double beta = 2.0 / (m * (psi + 1.0));
+        const double u = uniform_f64(wv);
+        V = p < u ? ::log((1.0 - p) / (1.0 - u)) / beta : 0.0;
+    }
+    const double k0 = -rho * kappa * theta / eps * delta + R.r_drift * delta;
+    const double k1 = 0.5 * delta * (kappa * rho / eps - 0.5) - rho / eps;
+    const double k2 = 0.5 * delta * (kappa * rho / eps - 0.5) + rho / eps;
+    const double k3 = 0.5 * delta * (1.0 - rho * rho);
+    const double k4 = 0.5 * delta * (1.0 - rho * rho);
+    logX = logX + k0 + k1 * Vp + k2 * V + ::sqrt(k3 * Vp + k4 * V) * normal_f64(wx);
+    return quad;
+}
+
+__device__ __forceinline__ bool central_f64(uint32_t k) { return __builtin_fabs(uniform_f64(k) - 0.5) <= 0.425; }
+
+template <bool DEBUG, bool COUNT>
+__device__ __forceinline__ void path_fp64(const PathArgs& a, uint32_t ri, uint32_t tile, int lane, double2* vals) {
+    const DevReq& R = a.reqs[ri];
+    const uint64_t pl = (uint64_t)tile * kTile + (uint64_t)lane;
+    const bool valid = pl < R.n_paths;
+    const uint64_t gp = R.path_offset + pl;
+    const uint32_t plo = (uint32_t)gp, phi = (uint32_t)(gp >> 32);
+    double V = R.v0, logX = R.log_S0, X = R.S0, Xp = R.S0, acc = 0.0;
+    uint32_t wb0 = 0, wb1 = 0, k = 0;
+    uint32_t cq = 0, ce = 0, cc = 0, ct = 0;
+    for (uint32_t s = 0; s < R.n_segs; ++s) {
+        const DevSeg& S = a.segs[R.seg_begin + s];
+        const double delta = S.delta;
+        const uint32_t n = S.n_steps;
+        for (uint32_t i = 0; i < n; ++i, ++k) {
+            uint32_t wv, wx;
+            if ((k & 1u) == 0u) {
+                const u32x4 w = philox4x32_10(k >> 1, plo, phi, 0u, R.seed_lo, R.seed_hi);
+                wv = w.x;
+                wx = w.y;
+                wb0 = w.z;
+                wb1 = w.w;
+            } else {
+                wv = wb0;
+                wx = wb1;
+            }
+            const bool quad = qe_step_fp64(R, delta, V, logX, wv, wx);
+            Xp = X;
+            X = ::exp(logX);
+            if constexpr (COUNT) {
+                if (valid) {
+                    cq += quad ? 1u : 0u;
+                    ce += quad ? 0u : 1u;
+                    const bool cv = central_f64(wv), cx = central_f64(wx);
+                    cc += (quad && cv ? 1u : 0u) + (cx ? 1u : 0u);
+                    ct += (quad && !cv ? 1u : 0u) + (cx ? 0u : 1u);
+                }
+            }
+            if (i + 1u < n) acc += delta * 0.5 * (X + Xp);  // AAsianCallNonAdaptive::accumulate_value
+        }
+        for (uint32_t f = 0; f < S.fin_count; ++f) {
+            const DevFin& F = a.fins[S.fin_begin + f];
+            const double xt = Xp + (X - Xp) * (F.T - F.tp) / F.dt;
+            const double A = (R.flags & kTrapezoid) ? (acc + (F.T - F.tp) * 0.5 * (Xp + xt)) / F.T
+                                                    : (acc + (X - Xp) * (F.T - F.tp) / F.dt) / F.T;
+            vals[lane] = make_double2(A, xt);
+            if constexpr (DEBUG) {
+                if (valid && a.finals) {
+                    double* out = a.finals + (pl * R.n_exp + F.chain) * 2;
+                    out[0] = A;
+                    out[1] = xt;
+                }
+            }
+            wave_sync_lds();
+            reduce_tiles<1>(a, R, F, tile, lane, vals, false);
+            wave_sync_lds();
+        }
+        acc += S.dnext * 0.5 * (X + Xp);
+    }
+    if constexpr (COUNT) count_flush(a.counts, cq, ce, cc, ct, lane);
+}
+
+template <bool DEBUG, bool COUNT>
+__global__ __launch_bounds__(256) void k_path_fp64(PathArgs a) {
+    extern __shared__ double2 lds_d2[];
+    const int lane = (int)(threadIdx.x & 63u);
+    double2* vals = lds_d2 + (threadIdx.x >> 6) * kTile;
+    for (;;) {
+        const uint32_t item = next_item(a.counter, lane);
+        if (item >= a.total_items) break;
+        const uint32_t sp = uniform(find_sorted(a.item_end, a.n_sorted, item));
+        const uint32_t first = sp ? a.item_end[sp - 1] : 0u;
+        const uint32_t ri = uniform(a.order[sp]);
+        path_fp64<DEBUG, COUNT>(a, ri, uniform(item - first), lane, vals);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Reduction: pass A sums each 64-tile group of a (request, output) in tile order; pass B sums
+// the groups in order and forms the estimator. Both orders depend only on the request.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t find_req(const uint64_t* end, uint32_t n, uint64_t t) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (end[mid] > t)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_groups(ReduceArgs a, uint64_t total) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const uint32_t ri = find_req(a.group_end, a.n_req, t);
+    const uint64_t local = t - (ri ? a.group_end[ri - 1] : 0ull);
+    const DevReq& R = a.reqs[ri];
+    const uint32_t n_out = R.n_opts * R.n_pay;
+    const uint32_t g = (uint32_t)(local / n_out);
+    const uint32_t o = (uint32_t)(local - (uint64_t)g * n_out);
+    const uint32_t t0 = g * kGroupTiles;
+    const uint32_t t1 = min(t0 + (uint32_t)kGroupTiles, R.n_tiles);
+    const size_t gslot = (size_t)R.group_base + (size_t)g * n_out + o;
+    if (R.precision == kPrecComp) {
+        const float4* p = reinterpret_cast<const float4*>(a.partials);
+        CompF s{0.f, 0.f}, ss{0.f, 0.f};
+        for (uint32_t tile = t0; tile < t1; ++tile) {
+            const float4 v = p[(size_t)R.partial_base + (size_t)tile * n_out + o];
+            s.merge(v.x, v.y);
+            ss.merge(v.z, v.w);
+        }
+        reinterpret_cast<float4*>(a.groups)[gslot] = make_float4(s.s, s.c, ss.s, ss.c);
+    } else {
+        const double2* p = reinterpret_cast<const double2*>(a.partials);
+        double s = 0.0, ss = 0.0;
+        for (uint32_t tile = t0; tile < t1; ++tile) {
+            const double2 v = p[(size_t)R.partial_base + (size_t)tile * n_out + o];
+            s += v.x;
+            ss += v.y;
+        }
+        reinterpret_cast<double2*>(a.groups)[gslot] = make_double2(s, ss);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(ReduceArgs a, uint64_t total) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const uint32_t ri = find_req(a.out_end, a.n_req, t);
+    const uint32_t o = (uint32_t)(t - (ri ? a.out_end[ri - 1] : 0ull));
+    const DevReq& R = a.reqs[ri];
+    const uint32_t n_out = R.n_opts * R.n_pay;
+    double s, ss;
+    if (R.precision == kPrecComp) {
+        const float4* p = reinterpret_cast<const float4*>(a.groups);
+        CompF cs{0.f, 0.f}, css{0.f, 0.f};
+        for (uint32_t g = 0; g < R.n_groups; ++g) {
+            const float4 v = p[(size_t)R.group_base + (size_t)g * n_out + o];
+            cs.merge(v.x, v.y);
+            css.merge(v.z, v.w);
+        }
+        s = (double)cs.s + (double)cs.c;
+        ss = (double)css.s + (double)css.c;
+    } else {
+        const double2* p = reinterpret_cast<const double2*>(a.groups);
+        s = 0.0;
+        ss = 0.0;
+        for (uint32_t g = 0; g < R.n_groups; ++g) {
+            const double2 v = p[(size_t)R.group_base + (size_t)g * n_out + o];
+            s += v.x;
+            ss += v.y;
+        }
+    }
+    const uint32_t opt = o % R.n_opts;
+    const double disc = a.opts[R.opt_begin + opt].disc;
+    const double scale = (R.precision == kPrecFP64) ? 1.0 : R.S0;
+    const double n = (double)R.n_paths;
+    const double mean = s / n;
+    double var = n > 1.0 ? (ss - n * mean * mean) / (n - 1.0) : 0.0;
+    var = var > 0.0 ? var : 0.0;
+    a.prices[R.out_base + o] = scale * disc * mean;
+    if (a.std_errs) a.std_errs[R.out_base + o] = scale * disc * ::sqrt(var / n);
+}
+
+// ------------------------------------------------------------------------------------------
+// Component kernels for the parity tests and measurement.
+// ------------------------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_rng_fill(uint32_t* out, uint64_t seed, uint32_t stream, uint64_t path0, uint64_t n_paths,
+                           uint64_t word0, uint64_t n_words) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_paths * n_words) return;
+    const uint64_t i = t / n_words, j = t - i * n_words;
+    const uint64_t w = word0 + j, path = path0 + i;
+    const u32x4 b = philox4x32_10((uint32_t)(w >> 2), (uint32_t)path, (uint32_t)(path >> 32), stream, (uint32_t)seed,
+                                  (uint32_t)(seed >> 32));
+    const uint32_t e = (uint32_t)(w & 3u);
+    out[t] = e == 0 ? b.x : e == 1 ? b.y : e == 2 ? b.z : b.w;
+}
+
+__global__ void k_philox(const uint32_t* ctr, const uint32_t* key, uint64_t n, uint32_t* out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u32x4 b = philox4x32_10(ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3], key[2 * i], key[2 * i + 1]);
+    out[4 * i] = b.x;
+    out[4 * i + 1] = b.y;
+    out[4 * i + 2] = b.z;
+    out[4 * i + 3] = b.w;
+}
+
+__global__ void k_normals(const uint32_t* words, uint64_t n, uint32_t precision, double* out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = precision == kPrecFP64 ? normal_f64(words[i]) : (double)normal_f32(words[i]);
+}
+
+__global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+inline dim3 grid_for(uint64_t n, uint32_t block) { return dim3((uint32_t)((n + block - 1) / block)); }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// Launch wrappers (host)
+// ------------------------------------------------------------------------------------------
+const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode) {
+    if (precision == kPrecFP64) {
+        switch (mode) {
+            case PathMode::Normal: return reinterpret_cast<const void*>(&k_path_fp64<false, false>);
+            case PathMode::Debug: return reinterpret_cast<const void*>(&k_path_fp64<true, false>);
+            case PathMode::Count: return reinterpret_cast<const void*>(&k_path_fp64<false, true>);
+        }
+        return nullptr;
+    }
+#define HEXO_PPT_CASE(P)                                                                           \
+    case P:                                                                                        \
+        switch (mode) {                                                                            \
+            case PathMode::Normal: return reinterpret_cast<const void*>(&k_path_fp32<P, false, false>); \
+            case PathMode::Debug: return reinterpret_cast<const void*>(&k_path_fp32<P, true, false>);   \
+            case PathMode::Count: return reinterpret_cast<const void*>(&k_path_fp32<P, false, true>);   \
+        }                                                                                          \
+        return nullptr;
+    switch (ppt) {
+        HEXO_PPT_CASE(1)
+        HEXO_PPT_CASE(2)
+        HEXO_PPT_CASE(4)
+        default: return nullptr;
+    }
+#undef HEXO_PPT_CASE
+}
+
+size_t path_lds_bytes(uint32_t precision, uint32_t ppt, uint32_t block) {
+    const size_t waves = block / kTile;
+    return precision == kPrecFP64 ? waves * kTile * sizeof(double2) : waves * ppt * kTile * sizeof(float2);
+}
+
+hipError_t launch_paths(uint32_t precision, uint32_t ppt, PathMode mode, uint32_t grid, uint32_t block,
+                        hipStream_t st, const PathArgs& a) {
+    const size_t lds = path_lds_bytes(precision, ppt, block);
+    if (precision == kPrecFP64) {
+        switch (mode) {
+            case PathMode::Normal: k_path_fp64<false, false><<<grid, block, lds, st>>>(a); break;
+            case PathMode::Debug: k_path_fp64<true, false><<<grid, block, lds, st>>>(a); break;
+            case PathMode::Count: k_path_fp64<false, true><<<grid, block, lds, st>>>(a); break;
+        }
+        return hipGetLastError();
+    }
+#define HEXO_LAUNCH(P)                                                                        \
+    switch (mode) {                                                                           \
+        case PathMode::Normal: k_path_fp32<P, false, false><<<grid, block, lds, st>>>(a); break; \
+        case PathMode::Debug: k_path_fp32<P, true, false><<<grid, block, lds, st>>>(a); break;   \
+        case PathMode::Count: k_path_fp32<P, false, true><<<grid, block, lds, st>>>(a); break;   \
+    }
+    switch (ppt) {
+        case 1: HEXO_LAUNCH(1) break;
+        case 2: HEXO_LAUNCH(2) break;
+        case 4: HEXO_LAUNCH(4) break;
+        default: return hipErrorInvalidValue;
+    }
+#undef HEXO_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const ReduceArgs& a, uint64_t n_group_threads, uint64_t n_out_threads, hipStream_t st) {
+    if (n_group_threads) k_reduce_groups<<<grid_for(n_group_threads, 256), 256, 0, st>>>(a, n_group_threads);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (n_out_threads) k_reduce_final<<<grid_for(n_out_threads, 256), 256, 0, st>>>(a, n_out_threads);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st) {
+    k_fill_u32<<<grid_for(n, 256), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st) {
+    k_fill_u64<<<grid_for(n, 256), 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rng_fill(uint32_t* out, uint64_t seed, uint32_t stream, uint64_t path0, uint64_t n_paths,
+                           uint64_t word0, uint64_t n_words, hipStream_t st) {
+    k_rng_fill<<<grid_for(n_paths * n_words, 256), 256, 0, st>>>(out, seed, stream, path0, n_paths, word0, n_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_philox(const uint32_t* ctr, const uint32_t* key, uint64_t n, uint32_t* out, hipStream_t st) {
+    k_philox<<<grid_for(n, 256), 256, 0, st>>>(ctr, key, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_normals(const uint32_t* words, uint64_t n, uint32_t precision, double* out, hipStream_t st) {
+    k_normals<<<grid_for(n, 256), 256, 0, st>>>(words, n, precision, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, uint32_t grid, hipStream_t st) {
+    k_copy<<<grid, 256, 0, st>>>(reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), bytes / 16);
+    return hipGetLastError();
+}
+
+}  // namespace hexo

@@ -1,0 +1,99 @@
+// layout.hpp -- HBM layout shared by the host engine and the path kernels.
+//
+// A batch is uploaded as one contiguous device arena:
+//   DevReq[n_req]        one record per pricing request (parameters, RNG key, table offsets)
+//   DevSeg[n_seg]        the request's time grid as runs of equal steps (host-built, path independent)
+//   DevFin[n_fin]        expiry finalisations at the end of a run
+//   DevOpt[n_opt]        strikes (relative and absolute) and discount factor per option
+//   uint32 order[n_req]  requests sorted longest-first; item_end[n_req] their work-item prefix sums
+//   partials             16 bytes per (64-path tile, output): FP64 (sum, sum of squares) or
+//                        compensated FP32 (sum, carry, sum of squares, carry)
+//   groups, results      per (64-tile group, output) partials; per output (price, std error)
+#pragma once
+#include <stdint.h>
+
+namespace hexo {
+
+constexpr int kTile = 64;         // paths per tile = one wavefront
+constexpr int kGroupTiles = 64;   // tiles per first-level reduction group
+constexpr int kMaxPPT = 4;        // paths per thread (FP32 kernel)
+
+// One run of steps with a constant step width (the reference's dt = T_c / steps between two
+// expiry finalisations). FP32 constants are computed on the host in FP64 (per request, per run):
+//   m  = V d + mth                   (conditional mean, src/HSimulation.tpp:58-59)
+//   s2 = V c1 + c2                   (conditional variance, src/HSimulation.tpp:60)
+//   y += k0 + kc (V + V') + kd (V' - V) + sqrt(k3 (V + V')) Z      (log2 of X/S0)
+// with kc = (K1 + K2)/2, kd = (K2 - K1)/2 = rho/eps, k3 = K3 = K4 (src/HSimulation.tpp:75-80),
+// all scaled by log2(e) (k3 by log2(e)^2).
+struct DevSeg {
+    float d, mth, c1, c2;
+    float k0, kc, kd, k3;
+    float delta_f;  // Asian weight of accumulations inside the run
+    float dnext_f;  // weight of the accumulation after the run's last step (new dt), 0 if none
+    uint32_t n_steps;
+    uint32_t fin_begin, fin_count;
+    uint32_t pad0;
+    double delta;  // FP64 kernel: step width
+    double dnext;  // FP64 kernel: dt after the finalisations
+};
+
+// Finalisation of one expiry (AAsianCallNonAdaptive / EuropeanCallNonAdaptive
+// accumulate_final_value, src/inc/AsianContract.h:29-34, src/inc/VanillaContract.h:28-31).
+struct DevFin {
+    uint32_t chain, opt_begin, n_strikes, pad0;
+    float w_f;     // (T - t_prev) / dt
+    float invT_f;  // 1 / T
+    float hwdt_f;  // (T - t_prev) / 2, corrected-trapezoid extension
+    float pad1;
+    double T, tp, dt;  // FP64 kernel: reference-form interpolation (X - Xp)(T - tp)/dt
+    double pad2;
+};
+
+struct DevOpt {
+    float k_rel;  // strike / S0 (FP32 kernel works in units of S0)
+    float pad0;
+    double k_abs;
+    double disc;  // exp(-r T) with HEXO_DISCOUNT, else 1
+    double pad1;
+};
+
+struct DevReq {
+    double v0, theta, rho, kappa, sigma, S0, r_drift, log_S0;
+    uint64_t n_paths, path_offset, partial_base, group_base;
+    uint32_t seed_lo, seed_hi;
+    uint32_t seg_begin, n_segs;
+    uint32_t n_opts, n_pay, payoffs, flags;
+    uint32_t precision, n_tiles, opt_begin, out_base;
+    uint32_t n_exp, n_groups;
+    float v0_f, pad0;
+};
+
+struct PathArgs {
+    const DevReq* reqs;
+    const DevSeg* segs;
+    const DevFin* fins;
+    const DevOpt* opts;
+    const uint32_t* order;     // sorted position -> request index
+    const uint32_t* item_end;  // sorted position -> exclusive prefix end of work items
+    uint32_t n_sorted;
+    uint32_t total_items;
+    uint32_t* counter;  // work-queue head, zeroed before each launch
+    void* partials;
+    double* finals;                 // optional per-path (A, X_T) per expiry
+    unsigned long long* counts;     // optional branch counters [4]
+};
+
+struct ReduceArgs {
+    const DevReq* reqs;
+    const DevOpt* opts;
+    const uint32_t* req_of;       // request index for sorted reduction slices
+    const uint64_t* group_end;    // prefix over requests of n_groups * n_out (pass A threads)
+    const uint64_t* out_end;      // prefix over requests of n_out (pass B threads)
+    uint32_t n_req;
+    const void* partials;
+    void* groups;
+    double* prices;
+    double* std_errs;
+};
+
+}  // namespace hexo
