@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Benchmark of the Heston QE Monte-Carlo pricing path on one MI355X.
+
+Workload (the metric's configuration): the strike/maturity grid -- Heston kappa=2, theta=0.04,
+sigma=0.3, rho=-0.7, v0=0.04, S0=100, r=0.02; 64 strikes in [70,130] x 8 maturities
+{0.1,...,5}; 1e6 paths per maturity shared by all strikes and by the Asian and European payoffs;
+round(52 T) steps; FP32 stepping with FP64 reduction. One step = one pricing pass over the grid.
+
+value = nominal path-steps/s (sum of n_paths * steps) of the FP32 batched path, inputs resident.
+Also reported: the plain FP64 kernel on the same workload and draws, the FP32-vs-FP64 price
+error, the path kernel's FP32 roofline fraction (algorithmic flops from exact branch counts), the
+batched uneven-request workload, serving latency, and the CPU baseline (the oracle, OpenMP).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+# Algorithmic FP32 flops of the path kernel (kernels.hip), convention: add/sub/mul = 1, fma = 2,
+# exp2/log2/sqrt/rcp = 1; compares, selects, conversions and integer (RNG) work are not counted.
+F_STEP = 19      # moments m, s2, m^2, 1.5 m^2 (6); log-price step incl. exp2 (12); Asian sum (1)
+F_QUAD = 14      # quadratic variance branch without its normal
+F_EXP = 12       # exponential variance branch incl. its uniform
+F_CENTRAL = 35   # one AS241 normal, central region
+F_TAIL = 38      # one AS241 normal, tail region
+F_SEG = 8        # per path and run of steps: Asian run sum folding and the new-dt accumulation
+F_FIN = 6        # per path and expiry: X_T interpolation and the Asian average
+F_PAYOFF = 1     # per path and output (FP32 subtraction; the sums are FP64)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_flops(reqs, counts) -> float:
+    n_quad, n_exp, n_central, n_tail = (float(c) for c in counts)
+    steps = n_quad + n_exp
+    f = F_STEP * steps + F_QUAD * n_quad + F_EXP * n_exp + F_CENTRAL * n_central + F_TAIL * n_tail
+    import hestonexotics_amd as H
+    for r in reqs:
+        tot, fin_step, _ = H.time_grid(r.expiries, r.n_strikes(), r.steps)
+        n_runs = len(set(int(s) for s in fin_step if s > 0))
+        f += r.n_paths * (F_SEG * n_runs - n_runs + F_FIN * len(r.expiries) + F_PAYOFF * r.n_outputs)
+    return f
+
+
+def cpu_baseline(reqs, target_s: float):
+    """The oracle (FP64, reference semantics, OpenMP) on a path subsample of the same requests."""
+    from oracle import oracle as O
+    lib = O.load("fast")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+
+    def run(scale):
+        done, secs = 0.0, 0.0
+        for r in reqs:
+            n = max(64, int(r.n_paths * scale))
+            out = O.price(r.params, r.S0, r.expiries, r.n_strikes(), r.flat_strikes(), steps=r.steps, n_paths=n,
+                          seed=r.seed, r=r.r, flags=r.flags, payoffs=r.payoffs, n_threads=threads, kind="fast")
+            secs += out["seconds"]
+            done += n * r.steps * len(r.expiries)
+        return done, secs
+
+    scale = 0.002
+    d, s = run(scale)
+    while s < 0.2 * target_s and scale < 1.0:  # grow the sample until its timing is meaningful
+        scale = min(1.0, scale * min(10.0, 0.5 * target_s / max(s, 1e-3)))
+        d, s = run(scale)
+    if s < 0.8 * target_s and scale < 1.0:
+        scale = min(1.0, scale * target_s / s)
+        d, s = run(scale)
+    return {"value": d / s, "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (FP64, reference semantics, {lib._hexo_name}) on {scale:.4f} of the paths of each "
+                      f"request: {d:.3e} path-steps in {s:.2f} s"}
+
+
+def time_batch(ctx, batch, steps: int, warmup: int):
+    for _ in range(warmup):
+        batch.run_eager()
+        batch.sync()
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run_eager()
+        batch.sync()
+        kernel_ms += batch.stats()["path_kernel_ms"]
+    wall = time.perf_counter() - t0
+    return wall, kernel_ms / steps
+
+
+def time_graph(batch, steps: int):
+    batch.run()
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+    batch.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--paths", type=int, default=1_000_000, help="paths per maturity")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-fp64-scale", type=float, default=0.05)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by paths or "
+              f"requests across GPUs with one small reduction; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import hestonexotics_amd as H
+    from hestonexotics_amd import workloads as W
+
+    ctx = H.Context(0)
+    info = ctx.device_info()
+    log("device", info)
+    reqs32 = W.c2(n_paths=args.paths, precision=H.FP32)
+    nominal = W.path_steps(reqs32)
+    if args.autotune:
+        log("autotune:", ctx.autotune(W.c2(n_paths=max(args.paths // 8, 64), precision=H.FP32), reps=2))
+    tuning = ctx.get_tuning()
+
+    b32 = ctx.batch(reqs32)
+    counts = b32.branch_counts()
+    flops = algorithmic_flops(reqs32, counts)
+    log("branch counts (quad, exp, central, tail):", counts.tolist(), "flops/pass %.4e" % flops)
+    wall32, kms32 = time_batch(ctx, b32, args.steps, args.warmup)
+    p32, s32 = b32.results()
+    graph_s = time_graph(b32, args.steps)
+    ms_per_step = 1e3 * wall32 / args.steps
+    value = nominal * args.steps / wall32
+    achieved = flops / (kms32 * 1e-3) / 1e12
+    log(f"FP32 C2: {ms_per_step:.3f} ms/pass (path kernel {kms32:.3f} ms), {value:.4e} path-steps/s, "
+        f"{achieved:.2f} TFLOP/s, graph replay {graph_s * 1e3:.3f} ms/pass")
+
+    # plain FP64 kernel on the same requests and draws
+    reqs64 = W.c2(n_paths=args.paths, precision=H.FP64)
+    b64 = ctx.batch(reqs64)
+    steps64 = max(2, args.steps // 10)
+    wall64, kms64 = time_batch(ctx, b64, steps64, 1)
+    p64, s64 = b64.results()
+    value64 = nominal * steps64 / wall64
+    big = p64 > 1e-2
+    rel = np.abs(p32[big] - p64[big]) / p64[big]
+    se_units = np.abs(p32 - p64) / np.maximum(s64, 1e-300)
+    log(f"FP64 C2: {1e3 * wall64 / steps64:.3f} ms/pass (kernel {kms64:.3f} ms), {value64:.4e} path-steps/s; "
+        f"FP32 vs FP64 max rel {rel.max():.3e} (prices > 0.01), max |diff|/SE {se_units.max():.3e}")
+
+    out = {
+        "metric": "Heston path-steps/s FP32 vs FP64 at <=1e-4 rel price error; p50/p99 latency",
+        "value": value,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 reduction)",
+        "data": "synthetic (seeded Heston strike/maturity grid)",
+        "config": {"workload": "C2 strike/maturity grid: 64 strikes x 8 maturities, 1e6 paths per maturity, "
+                               "52 steps/yr, Asian+European from shared paths",
+                   "paths_per_maturity": args.paths, "nominal_path_steps_per_step": nominal,
+                   "simulated_path_steps_per_step": b32.stats()["path_steps"], "tuning": tuning,
+                   "device": info},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_path_fp32", "kernel_ms": kms32, "algorithmic_flops_per_launch": flops,
+                     "branch_counts": {"quadratic_steps": int(counts[0]), "exponential_steps": int(counts[1]),
+                                       "central_normals": int(counts[2]), "tail_normals": int(counts[3])}},
+        "fp64_plain": {"value": value64, "ms_per_step": 1e3 * wall64 / steps64, "kernel_ms": kms64,
+                       "fp32_speedup": value / value64},
+        "fp32_vs_fp64": {"max_rel_err": float(rel.max()), "mean_rel_err": float(rel.mean()),
+                         "max_abs_err_in_se": float(se_units.max()), "n_prices": int(big.sum())},
+        "graph_replay_ms_per_step": graph_s * 1e3,
+    }
+
+    if not args.no_c3:
+        c3 = W.c3(precision=H.FP32)
+        b3 = ctx.batch(c3)
+        n3 = W.path_steps(c3)
+        w3, k3 = time_batch(ctx, b3, 1, 1)
+        p3, _ = b3.results()
+        c3d = W.c3(precision=H.FP64, path_scale=args.c3_fp64_scale)
+        c3f = W.c3(precision=H.FP32, path_scale=args.c3_fp64_scale)
+        b3d = ctx.batch(c3d)
+        w3d, k3d = time_batch(ctx, b3d, 1, 0)
+        pd, sd = b3d.results()
+        b3f = ctx.batch(c3f)
+        b3f.run_eager()
+        b3f.sync()
+        pf, _ = b3f.results()
+        bigd = pd > 1e-2
+        rel3 = np.abs(pf[bigd] - pd[bigd]) / pd[bigd]
+        v3, v3d = n3 / w3, W.path_steps(c3d) / w3d
+        out["c3_batched"] = {"requests": len(c3), "nominal_path_steps": n3, "fp32_value": v3, "fp32_s": w3,
+                             "fp32_kernel_ms": k3, "fp64_value": v3d, "fp64_path_scale": args.c3_fp64_scale,
+                             "fp32_over_fp64": v3 / v3d, "fp32_vs_fp64_max_rel_err": float(rel3.max())}
+        log("C3:", out["c3_batched"])
+
+    if args.serve_seconds > 0:
+        reqs4, arr = W.c4(duration_s=args.serve_seconds, precision=H.FP32)
+        ctx.server_start(streams=4)
+        t0 = time.perf_counter()
+        rc, p4, s4, infos = ctx.run_trace(reqs4, arr)
+        wall4 = time.perf_counter() - t0
+        st = ctx.server_stats()
+        ctx.server_stop()
+        lat = np.array([i["done_s"] - i["arrival_s"] for i in infos])
+        ok = np.array([i["status"] == 0 for i in infos])
+        out["serving"] = {"rc": rc, "requests": len(reqs4), "ok": int(ok.sum()), "rate_req_s": 200.0,
+                          "duration_s": args.serve_seconds, "wall_s": wall4,
+                          "offered_path_steps_per_s": W.path_steps(reqs4) / args.serve_seconds,
+                          "achieved_path_steps_per_s": W.path_steps(reqs4) / wall4,
+                          "p50_latency_ms": float(np.percentile(lat[ok], 50) * 1e3) if ok.any() else None,
+                          "p99_latency_ms": float(np.percentile(lat[ok], 99) * 1e3) if ok.any() else None,
+                          "batches": st["batches"], "max_queue_depth": st["max_queue_depth"]}
+        log("serving:", out["serving"])
+
+    if not args.no_cpu:
+        cb = cpu_baseline(W.c2(n_paths=args.paths, precision=H.FP64), args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        log("cpu baseline:", cb)
+    for bt in (b32, b64):
+        bt.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

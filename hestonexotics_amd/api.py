@@ -9,6 +9,7 @@ same paths, the precision and the documented extensions (drift, discount, correc
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 from typing import Sequence
 
@@ -187,6 +188,7 @@ class Context:
     def __init__(self, device: int = 0):
         self.lib = _lib.load()
         self.h = C.c_void_p()
+        self._batches = weakref.WeakSet()
         rc = self.lib.hexo_ctx_create(device, C.byref(self.h))
         if rc != 0:
             raise HexoError(rc, "hexo_ctx_create")
@@ -197,6 +199,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for b in list(self._batches):
+                b.close()
             self.lib.hexo_ctx_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -232,7 +236,9 @@ class Context:
         return out
 
     def batch(self, reqs: Sequence[Request]) -> Batch:
-        return Batch(self, reqs)
+        b = Batch(self, reqs)
+        self._batches.add(b)
+        return b
 
     def paths(self, req: Request, first: int, n: int) -> np.ndarray:
         out = np.zeros((n, len(req.expiries), 2), np.float64)

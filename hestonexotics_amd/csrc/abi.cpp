@@ -124,6 +124,14 @@ int hexo_ctx_destroy(hexo_ctx* ctx) {
     if (!ctx) return HEXO_ERR_INVALID;
     if (ctx->server) (void)hexo_server_stop(ctx);
     (void)hipSetDevice(ctx->device);
+    {
+        // batches still alive lose their device state here; their handles stay valid for
+        // hexo_batch_destroy, which then only frees the handle
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        for (hexo_batch* hb : ctx->batches) hb->b.reset();
+        ctx->batches.clear();
+    }
     if (ctx->stream) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
@@ -180,6 +188,7 @@ int hexo_batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hexo_
         delete hb;
         return ctx->set_error(rc, err);
     }
+    ctx->batches.insert(hb);
     *out = hb;
     return HEXO_OK;
 }
@@ -258,6 +267,7 @@ int hexo_batch_destroy(hexo_batch* hb) {
         DeviceGuard dg(ctx->device);
         (void)hipStreamSynchronize(hb->b->stream);
         hb->b.reset();
+        ctx->batches.erase(hb);
     }
     delete hb;
     return HEXO_OK;

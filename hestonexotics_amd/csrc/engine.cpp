@@ -267,8 +267,26 @@ size_t batch_device_bytes(const hexo_request* reqs, uint32_t n) {
     return s;
 }
 
+bool PinnedBuf::reserve(size_t bytes) {
+    if (bytes <= cap) return true;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes, (size_t)1 << 20);
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), want, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        return false;
+    }
+    cap = want;
+    return true;
+}
+
+PinnedBuf::~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+}
+
 int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_t stream, Arena* arena,
-                 std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n) {
+                 std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n, PinnedBuf* stage) {
     if (n == 0 || !reqs) {
         err = "empty batch";
         return HEXO_ERR_INVALID;
@@ -308,7 +326,15 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     // host staging: the uploaded tables, then the result buffers
     b->upload_bytes = L.upload_end;
     b->h_stage_bytes = L.upload_end + 2 * sizeof(double) * n_out;
-    if (hipHostMalloc(reinterpret_cast<void**>(&b->h_stage), b->h_stage_bytes, hipHostMallocDefault) != hipSuccess) {
+    if (stage) {
+        if (!stage->reserve(b->h_stage_bytes)) {
+            err = "hipHostMalloc of the staging buffer failed";
+            return HEXO_ERR_NOMEM;
+        }
+        b->h_stage = stage->p;
+        b->h_external = true;
+    } else if (hipHostMalloc(reinterpret_cast<void**>(&b->h_stage), b->h_stage_bytes, hipHostMallocDefault) !=
+               hipSuccess) {
         b->h_stage = nullptr;
         err = "hipHostMalloc of the staging buffer failed";
         return HEXO_ERR_NOMEM;
@@ -628,7 +654,7 @@ void batch_destroy(Batch& b) {
             (void)hipFree(b.d_mem);
     }
     b.d_mem = nullptr;
-    if (b.h_stage) (void)hipHostFree(b.h_stage);
+    if (b.h_stage && !b.h_external) (void)hipHostFree(b.h_stage);
     b.h_stage = nullptr;
 }
 

@@ -12,6 +12,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/hexo.h"
@@ -59,6 +60,15 @@ class Arena {
     std::unordered_map<size_t, size_t> live_;      // offset -> size
 };
 
+// Reusable pinned host buffer (serving: one per stream slot, so no pinned allocation or free --
+// which may synchronise the device -- happens per batch).
+struct PinnedBuf {
+    char* p = nullptr;
+    size_t cap = 0;
+    bool reserve(size_t bytes);
+    ~PinnedBuf();
+};
+
 // ---- a batch of requests resident on the device --------------------------------------------
 struct Batch {
     hexo_ctx* ctx = nullptr;
@@ -74,6 +84,7 @@ struct Batch {
     // host pinned staging
     char* h_stage = nullptr;
     size_t h_stage_bytes = 0;
+    bool h_external = false;  // h_stage belongs to a PinnedBuf
     size_t upload_bytes = 0;
     double* h_prices = nullptr;
     double* h_ses = nullptr;
@@ -107,6 +118,7 @@ struct hexo_ctx {
     std::recursive_mutex mu;
     std::string last_error;
     struct hexo_server_state* server = nullptr;  // owned; see server.cpp
+    std::unordered_set<struct hexo_batch*> batches;  // live batches; released by hexo_ctx_destroy
     int set_error(int code, const std::string& msg) {
         last_error = msg;
         return code;
@@ -122,7 +134,8 @@ namespace hexo {
 // Creates a batch: validates, builds the schedules, packs and uploads the tables, plans the
 // launches. mem/arena: when arena is non-null the device region comes from it.
 int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_t stream, Arena* arena,
-                 std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n = nullptr);
+                 std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n = nullptr,
+                 PinnedBuf* stage = nullptr);
 // Device bytes a set of requests needs (admission accounting).
 size_t batch_device_bytes(const hexo_request* reqs, uint32_t n);
 size_t request_device_bytes(const hexo_request* r);

@@ -52,11 +52,13 @@ __device__ __forceinline__ uint32_t find_sorted(const uint32_t* item_end, uint32
     return lo;
 }
 
+// Takes the wave's next work item. Every lane executes the atomic (lane 0 adds 1, the others 0)
+// and the wave keeps lane 0's ticket, so there is no divergent region around the atomic: with
+// `if (lane == 0) atomicAdd(...)` followed by a broadcast, the loop structurizer may move the
+// broadcast into a loop that lanes 1..63 re-enter without lane 0, repeating one item forever.
 __device__ __forceinline__ uint32_t next_item(uint32_t* counter, int lane) {
-    uint32_t item = 0;
-    if (lane == 0) item = atomicAdd(counter, 1u);
-    item = __shfl(item, 0);
-    return uniform(item);
+    const uint32_t old = atomicAdd(counter, lane == 0 ? 1u : 0u);
+    return uniform(old);
 }
 
 // Compensated (Neumaier) FP32 accumulation.
@@ -180,12 +182,12 @@ __device__ __forceinline__ void count_flush(unsigned long long* counts, uint32_t
         c2 += __shfl_xor(c2, m);
         c3 += __shfl_xor(c3, m);
     }
-    if (lane == 0) {
-        atomicAdd(&counts[0], (unsigned long long)c0);
-        atomicAdd(&counts[1], (unsigned long long)c1);
-        atomicAdd(&counts[2], (unsigned long long)c2);
-        atomicAdd(&counts[3], (unsigned long long)c3);
-    }
+    // all lanes hold the wave total after the butterfly; only lane 0 contributes it (no divergent region)
+    const bool first = lane == 0;
+    atomicAdd(&counts[0], first ? (unsigned long long)c0 : 0ull);
+    atomicAdd(&counts[1], first ? (unsigned long long)c1 : 0ull);
+    atomicAdd(&counts[2], first ? (unsigned long long)c2 : 0ull);
+    atomicAdd(&counts[3], first ? (unsigned long long)c3 : 0ull);
 }
 
 // ------------------------------------------------------------------------------------------

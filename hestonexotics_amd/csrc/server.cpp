@@ -38,6 +38,7 @@ struct hexo_server_state {
     hexo::Arena arena;
     std::vector<hipStream_t> streams;
     std::vector<bool> slot_busy;
+    std::vector<std::unique_ptr<hexo::PinnedBuf>> stages;  // one pinned staging buffer per stream slot
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv;
@@ -137,7 +138,8 @@ void hexo_server_state::launch_ready() {
         }
         std::unique_ptr<hexo::Batch> b;
         std::string err;
-        int rc = hexo::batch_create(ctx, reqs.data(), (uint32_t)reqs.size(), streams[slot], &arena, b, err);
+        int rc = hexo::batch_create(ctx, reqs.data(), (uint32_t)reqs.size(), streams[slot], &arena, b, err, nullptr,
+                                    stages[slot].get());
         if (rc == HEXO_ERR_NOMEM && !flights.empty()) {
             // fragmentation: put the requests back and retry when a batch completes
             for (auto it = ids.rbegin(); it != ids.rend(); ++it) queue.push_front(*it);
@@ -220,7 +222,7 @@ int hexo_server_start(hexo_ctx* ctx, const hexo_server_config* cfg) {
     if (budget == 0) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
-        budget = std::min<size_t>(fr / 4, (size_t)32 << 30);
+        budget = std::min<size_t>(fr / 4, (size_t)8 << 30);
     }
     if (!s->arena.init(budget)) {
         delete s;
@@ -230,6 +232,7 @@ int hexo_server_start(hexo_ctx* ctx, const hexo_server_config* cfg) {
     s->st.budget_bytes = s->budget;
     s->streams.resize(s->cfg.streams);
     s->slot_busy.assign(s->cfg.streams, false);
+    for (uint32_t i = 0; i < s->cfg.streams; ++i) s->stages.push_back(std::make_unique<hexo::PinnedBuf>());
     for (auto& st : s->streams)
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
             st = nullptr;

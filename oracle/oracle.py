@@ -55,11 +55,14 @@ _LIBS: dict[str, C.CDLL] = {}
 
 
 def load(kind: str = "strict") -> C.CDLL:
-    """kind: 'strict' (checker build) or 'fast' (reference numerics flags, for timing)."""
+    """kind: 'strict' (checker build), 'fma' (same source with FMA contraction) or 'fast'
+    (reference numerics flags, for timing)."""
     if kind in _LIBS:
         return _LIBS[kind]
     if kind == "strict":
         name = "liboracle.so"
+    elif kind == "fma":
+        name = "liboracle_fma.so"
     else:
         flags = ""
         try:
@@ -76,7 +79,7 @@ def load(kind: str = "strict") -> C.CDLL:
     if not os.path.exists(path):
         raise RuntimeError(f"oracle library {path} is not built; run __graft_entry__.build()")
     lib = _bind(C.CDLL(path))
-    if kind == "strict":
+    if kind in ("strict", "fma"):
         lib.orc_swift_price.argtypes = [_f64p, C.c_double, C.c_double, C.c_double, C.c_uint32, _f64p, C.c_void_p,
                                         C.c_double, _f64p, _f64p]
         lib.orc_swift_price.restype = C.c_int

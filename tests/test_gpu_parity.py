@@ -75,23 +75,34 @@ def small_requests():
     }
 
 
-def oracle_for(oracle, req, n_paths=None, want_finals=True):
+def oracle_for(oracle, req, n_paths=None, want_finals=True, kind="strict"):
     return oracle.price(req.params, req.S0, req.expiries, req.n_strikes(), req.flat_strikes(), steps=req.steps,
                         n_paths=req.n_paths if n_paths is None else n_paths, seed=req.seed, r=req.r,
                         flags=req.flags, payoffs=req.payoffs, path_offset=req.path_offset,
-                        want_finals=want_finals, want_counts=True)
+                        want_finals=want_finals, want_counts=True, kind=kind)
 
 
 @pytest.mark.parametrize("name", list(small_requests().keys()))
 def test_pathwise_fp64(ctx, oracle, name):
+    """1e-12 relative, or twice the spread between two valid FP64 evaluations of the same formula
+    (the oracle with and without FMA contraction) where the QE recursion amplifies last-bit
+    differences (the stiff case: sigma = 0.9, kappa = 0.5, frequent near-zero variance)."""
     req = small_requests()[name]
     say("gpu pathwise fp64", name)
     g = ctx.paths(req, 0, req.n_paths)
     o = oracle_for(oracle, req)["finals"]
+    o_fma = oracle_for(oracle, req, kind="fma")["finals"]
     mask = ~np.isnan(o)
     assert np.array_equal(mask, ~np.isnan(g)), "finalised expiries differ"
+    spread = (np.abs(o_fma[mask] - o[mask]) / np.abs(o[mask])).max()
+    tol = max(1e-12, 2.0 * spread)
     rel = np.abs(g[mask] - o[mask]) / np.abs(o[mask])
-    assert rel.max() < 1e-12, f"{name}: max rel {rel.max():.3e}"
+    rel_fma = np.abs(g[mask] - o_fma[mask]) / np.abs(o_fma[mask])
+    say(f"  max rel vs oracle {rel.max():.3e}, vs fma oracle {rel_fma.max():.3e}, oracle spread {spread:.3e}")
+    assert rel.max() < tol, f"{name}: max rel {rel.max():.3e} (tol {tol:.1e})"
+    assert rel_fma.max() < tol, f"{name}: max rel vs fma oracle {rel_fma.max():.3e}"
+    if name != "stiff_c5":
+        assert tol == 1e-12
     # payoffs at every strike, absolute tolerance 1e-12 * A (max(0, A - K) cancels near the money)
     ns = req.n_strikes()
     K = req.flat_strikes()
@@ -167,3 +178,53 @@ def test_tuning_does_not_change_results(ctx):
                 np.testing.assert_array_equal(s, t)
     finally:
         ctx.set_tuning(**base)
+
+
+def test_branch_counts(ctx, oracle):
+    """Counting variant of the kernels: the FP64 kernel's branch decisions equal the oracle's; the
+    FP32 kernel's differ only where FP32 rounding moves psi across 1.5; step and normal totals are
+    exact."""
+    say("gpu branch counts")
+    reqs = list(small_requests().values())
+    b = ctx.batch(reqs)
+    c64 = b.branch_counts()
+    b.close()
+    quad = exp = 0
+    for r in reqs:
+        o = oracle_for(oracle, r, want_finals=False)
+        quad += int(o["counts"][0])
+        exp += int(o["counts"][1])
+    assert int(c64[0]) == quad and int(c64[1]) == exp
+    assert int(c64[2] + c64[3]) == 2 * quad + exp
+    for r in reqs:
+        r.precision = H.FP32
+    b32 = ctx.batch(reqs)
+    c32 = b32.branch_counts()
+    b32.close()
+    assert int(c32[0] + c32[1]) == quad + exp
+    assert abs(int(c32[0]) - quad) <= 1e-3 * (quad + exp)
+    assert int(c32[2] + c32[3]) == 2 * int(c32[0]) + int(c32[1])
+
+
+def test_graph_replays_match_eager(ctx):
+    """The captured launch sequence, replayed several times with other eager work in between,
+    gives the eager results every time."""
+    say("gpu graph replays")
+    reqs = W.c2(n_paths=20_000, maturities=[0.25, 1.0], n_strikes=8)
+    other = small_requests()["c1_like"]
+    other.precision = H.FP32
+    eager = ctx.price_batch(reqs)
+    b = ctx.batch(reqs)
+    for rep in range(4):
+        b.run()
+        b.sync()
+        p, s = b.results()
+        for (pe, se), pg, sg in zip(eager, b.per_request(p), b.per_request(s)):
+            np.testing.assert_array_equal(pe, pg)
+            np.testing.assert_array_equal(se, sg)
+        ctx.price(other)  # eager work between replays
+        b.run_eager()
+        b.sync()
+        p2, _ = b.results()
+        np.testing.assert_array_equal(p, p2)
+    b.close()

@@ -1,0 +1,119 @@
+"""Serving loop on the GPU: continuous batching, admission by device memory, failure handling.
+
+Each request's result is the same alone (hexo_price) and in the mix (served in whatever batch it
+landed in), bit for bit: its tiles, reduction order and RNG stream depend only on the request.
+"""
+import numpy as np
+import pytest
+
+import hestonexotics_amd as H
+from hestonexotics_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def small_trace(n=40, seed=3):
+    reqs, arr = W.c4(rate=400.0, duration_s=n / 400.0 * 1.2, seed=seed, path_scale=0.005)
+    reqs = reqs[:n]
+    arr = arr[:n]
+    return reqs, arr
+
+
+def test_served_equals_alone(ctx):
+    reqs, arr = small_trace()
+    say("gpu serving: trace of", len(reqs), "requests")
+    ctx.server_start(streams=3)
+    try:
+        rc, p, s, infos = ctx.run_trace(reqs, arr)
+        st = ctx.server_stats()
+    finally:
+        ctx.server_stop()
+    assert rc == 0, rc
+    assert st["completed"] == len(reqs)
+    o = 0
+    for r, inf in zip(reqs, infos):
+        p1, s1 = ctx.price(r)
+        n = r.n_outputs
+        np.testing.assert_array_equal(p[o:o + n], p1)
+        np.testing.assert_array_equal(s[o:o + n], s1)
+        assert inf["status"] == 0 and inf["done_s"] >= inf["launch_s"] >= inf["admit_s"] >= inf["arrival_s"]
+        o += n
+    say("  batches:", st["batches"], "max batch:", max(i["batch_size"] for i in infos))
+
+
+def test_submit_wait_poll_cancel(ctx):
+    reqs, _ = small_trace(8, seed=5)
+    ctx.server_start(streams=1)
+    try:
+        tickets = [ctx.submit(r) for r in reqs]
+        # the last tickets are likely still queued behind the first batch; cancel one if so
+        cancelled = None
+        for t in reversed(tickets):
+            try:
+                ctx.cancel(t)
+                cancelled = t
+                break
+            except H.HexoError:
+                continue
+        for r, t in zip(reqs, tickets):
+            if t == cancelled:
+                with pytest.raises(H.HexoError):
+                    ctx.wait(t, r.n_outputs)
+                continue
+            p, s, info = ctx.wait(t, r.n_outputs, timeout_s=60.0)
+            p1, _ = ctx.price(r)
+            np.testing.assert_array_equal(p, p1)
+        with pytest.raises(H.HexoError):
+            ctx.poll(10**9)
+    finally:
+        ctx.server_stop()
+
+
+def test_admission_refused_by_budget(ctx):
+    big = W.c1(n_paths=5_000_000, precision=H.FP32)
+    big.strikes = [list(np.linspace(50, 150, 400))]
+    ctx.server_start(memory_budget_bytes=1 << 20, streams=1)
+    try:
+        with pytest.raises(H.HexoError) as e:
+            ctx.submit(big)
+        assert e.value.code == -5  # HEXO_ERR_ADMISSION
+        small = W.c1(n_paths=1000, precision=H.FP32)
+        t = ctx.submit(small)
+        p, s, _ = ctx.wait(t, small.n_outputs, timeout_s=60.0)
+        assert np.isfinite(p).all()
+    finally:
+        ctx.server_stop()
+
+
+def test_malformed_request_rejected(ctx):
+    bad = W.c1(n_paths=1000)
+    bad.steps = 0
+    ctx.server_start(streams=1)
+    try:
+        with pytest.raises(H.HexoError) as e:
+            ctx.submit(bad)
+        assert e.value.code == -1
+    finally:
+        ctx.server_stop()
+
+
+def test_injected_launch_failure_is_reported_and_server_recovers(ctx):
+    reqs, _ = small_trace(4, seed=9)
+    ctx.server_start(streams=1, max_batch=1)
+    try:
+        ctx.inject_fault(0, 1)
+        tickets = [ctx.submit(r) for r in reqs]
+        codes = []
+        for r, t in zip(reqs, tickets):
+            try:
+                ctx.wait(t, r.n_outputs, timeout_s=60.0)
+                codes.append(0)
+            except H.HexoError as e:
+                codes.append(e.code)
+        assert codes[0] == -10 and all(c == 0 for c in codes[1:]), codes
+    finally:
+        ctx.server_stop()
