@@ -120,6 +120,10 @@ def main():
     ap.add_argument("--c3-fp64-scale", type=float, default=0.05)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiling driver: 8 eager FP32 passes of the workload, then a plain copy of "
+                         "--copy-mib MiB (the known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--copy-mib", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by paths or "
@@ -132,6 +136,17 @@ def main():
     ctx = H.Context(0)
     info = ctx.device_info()
     log("device", info)
+    if args.profile:
+        bp = ctx.batch(W.c2(n_paths=args.paths, precision=H.FP32))
+        for i in range(8):
+            bp.run_eager()
+            bp.sync()
+            log(f"profile pass {i}: path kernel {bp.stats()['path_kernel_ms']:.3f} ms")
+        bp.close()
+        ms = ctx.copy_probe(args.copy_mib << 20, reps=3)
+        log(f"copy probe: {args.copy_mib} MiB in {ms:.3f} ms ({(2 * (args.copy_mib << 20)) / ms / 1e6:.1f} GB/s)")
+        ctx.close()
+        return
     reqs32 = W.c2(n_paths=args.paths, precision=H.FP32)
     nominal = W.path_steps(reqs32)
     if args.autotune:

@@ -489,3 +489,41 @@ int orc_price(const double params[5], double S0, double r, uint32_t flags, uint3
     free(opt_off);
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Moments of one QE variance step from state V over n independent draws (draw i uses path i of
+ * the stream, words 0 and 1). out = {mean, variance, 4th central moment, fraction V' = 0,
+ * fraction of quadratic-branch draws, mean of exp(log-price increment)}.
+ * ------------------------------------------------------------------------------------------ */
+void orc_qe_moments(const double params[5], double delta, double V, uint64_t n, uint64_t seed, double out[6]) {
+    double s1 = 0, s2 = 0, s3 = 0, s4 = 0, zeros = 0, quads = 0, ex = 0;
+    double vals_mean = 0.0;
+    /* two passes for a stable central moment: first the mean */
+    for (int pass = 0; pass < 2; ++pass) {
+        s1 = s2 = s3 = s4 = zeros = quads = ex = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            uint32_t blk[4];
+            stream_block(seed, 0u, i, 0, blk);
+            double v = V, lx = 0.0;
+            int q = orc_qe_step(params, 0.0, delta, &v, &lx, blk[0], blk[1]);
+            if (pass == 0) {
+                s1 += v;
+                continue;
+            }
+            const double d = v - vals_mean;
+            s2 += d * d;
+            s4 += d * d * d * d;
+            s3 += v;
+            zeros += v == 0.0;
+            quads += q;
+            ex += exp(lx);
+        }
+        if (pass == 0) vals_mean = s1 / (double)n;
+    }
+    out[0] = vals_mean;
+    out[1] = s2 / (double)(n - 1);
+    out[2] = s4 / (double)n;
+    out[3] = zeros / (double)n;
+    out[4] = quads / (double)n;
+    out[5] = ex / (double)n;
+}

@@ -79,6 +79,8 @@ int orc_price(const double params[5], double S0, double r, uint32_t flags, uint3
               const uint32_t* n_strikes, const double* strikes, double* prices, double* std_errs, double* finals,
               uint64_t* counts, int n_threads);
 
+void orc_qe_moments(const double params[5], double delta, double V, uint64_t n, uint64_t seed, double out[6]);
+
 /* Wall-clock of the last orc_price call (seconds, steady clock, pricing loop only). */
 double orc_last_seconds(void);
 int orc_max_threads(void);

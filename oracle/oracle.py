@@ -195,3 +195,13 @@ def heston_chf(params, tau: float, u: float) -> complex:
     out = np.zeros(2, np.float64)
     load().orc_heston_chf(np.asarray(params, np.float64), tau, u, out)
     return complex(out[0], out[1])
+
+
+def qe_moments(params, delta: float, V: float, n: int, seed: int = 1) -> dict:
+    lib = load()
+    if not hasattr(lib, "_qe_bound"):
+        lib.orc_qe_moments.argtypes = [_f64p, C.c_double, C.c_double, C.c_uint64, C.c_uint64, _f64p]
+        lib._qe_bound = True
+    out = np.zeros(6, np.float64)
+    lib.orc_qe_moments(np.asarray(params, np.float64), delta, V, n, seed, out)
+    return dict(zip(["mean", "var", "m4", "zero_frac", "quad_frac", "mean_exp_dlogx"], out.tolist()))

@@ -1,0 +1,117 @@
+"""The C ABI without a GPU: the library loads, exports every entry point include/hexo.h declares,
+and its host-only functions (validation, output count, time grid, error strings) behave."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import hestonexotics_amd as H
+from hestonexotics_amd import _lib, workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "hexo.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|uint64_t|const char\*)\s+(hexo_[a-z0-9_]+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def test_version_and_errors():
+    lib = _lib.load()
+    assert b"gfx950" in lib.hexo_version()
+    for code in range(0, -11, -1):
+        assert lib.hexo_strerror(code) != b"unknown status"
+    assert lib.hexo_strerror(-99) == b"unknown status"
+
+
+def good():
+    return W.c1(n_paths=1000)
+
+
+MUTATIONS = {
+    "steps": lambda r: setattr(r, "steps", 0),
+    "sigma": lambda r: setattr(r, "sigma", 0.0),
+    "kappa": lambda r: setattr(r, "kappa", -1.0),
+    "theta": lambda r: setattr(r, "theta", 0.0),
+    "v0": lambda r: setattr(r, "v0", -0.01),
+    "rho": lambda r: setattr(r, "rho", 1.5),
+    "S0": lambda r: setattr(r, "S0", 0.0),
+    "paths": lambda r: setattr(r, "n_paths", 0),
+    "expiry_order": lambda r: (setattr(r, "expiries", [1.0, 0.5]), setattr(r, "strikes", [[100.0], [100.0]])),
+    "expiry_nan": lambda r: setattr(r, "expiries", [float("nan")]),
+    "strike_nan": lambda r: setattr(r, "strikes", [[float("inf")]]),
+    "no_strikes": lambda r: setattr(r, "strikes", [[]]),
+    "payoffs": lambda r: setattr(r, "payoffs", 0),
+    "payoff_bits": lambda r: setattr(r, "payoffs", 8),
+    "flags": lambda r: setattr(r, "flags", 64),
+    "precision": lambda r: setattr(r, "precision", 7),
+}
+
+
+def test_validation_accepts_good_request():
+    assert H.validate(good()) == 0
+    r = good()
+    r.expiries, r.strikes = [0.3, 0.31, 1.0], [[90, 100], [], [100]]
+    assert H.validate(r) == 0
+
+
+@pytest.mark.parametrize("name", sorted(MUTATIONS))
+def test_validation_rejects_malformed(name):
+    r = good()
+    MUTATIONS[name](r)
+    assert H.validate(r) == -1
+
+
+def test_reserved_field_must_be_zero():
+    lib = _lib.load()
+    q = good().to_c()
+    q.reserved = 1
+    assert lib.hexo_validate_request(C.byref(q)) == -1
+
+
+def test_n_outputs():
+    lib = _lib.load()
+    r = good()
+    r.expiries, r.strikes, r.payoffs = [0.5, 1.0], [[90, 100, 110], [100]], H.ASIAN | H.EUROPEAN
+    assert lib.hexo_request_n_outputs(C.byref(r.to_c())) == 8
+    r.payoffs = H.EUROPEAN
+    assert lib.hexo_request_n_outputs(C.byref(r.to_c())) == 4
+    r.steps = 0
+    assert lib.hexo_request_n_outputs(C.byref(r.to_c())) == 0
+
+
+def test_time_grid_rejects_bad_input():
+    with pytest.raises(H.HexoError):
+        H.time_grid([1.0, 0.5], [1, 1], 10)
+    with pytest.raises(H.HexoError):
+        H.time_grid([1.0], [0], 10)
+
+
+def test_context_without_gpu_reports_no_device():
+    import subprocess, sys
+    # run in a child so that a GPU-less HIP initialisation cannot affect this process
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import hestonexotics_amd as H\n"
+            "try:\n    H.Context(0); print('ok')\nexcept H.HexoError as e:\n    print(e.code)\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120).stdout.strip()
+    assert out in ("ok", "-2"), out
+
+
+def test_workload_generators_are_seeded():
+    a = W.c3(n=16, seed=3, path_scale=0.01)
+    b = W.c3(n=16, seed=3, path_scale=0.01)
+    assert [(r.kappa, r.n_paths, r.seed) for r in a] == [(r.kappa, r.n_paths, r.seed) for r in b]
+    reqs, t = W.c4(duration_s=2.0, seed=1)
+    assert np.all(np.diff(t) > 0) and 300 < len(reqs) < 500
+    assert W.path_steps(W.c2()) == 824_000_000

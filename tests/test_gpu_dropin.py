@@ -1,0 +1,52 @@
+"""The drop-in boundary on the GPU: the `hexo -p asian all` CLI (reference grammar and output
+columns, src/Main.cpp:75-96) and the header-only HSimulation::price<HQEAnderson<...>> wrapper it
+is built on give the C ABI's prices for the same request."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import hestonexotics_amd as H
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "hestonexotics_amd", "csrc", "hexo")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def run_cli(*args, env_extra=None):
+    env = dict(os.environ, HEXO_DATA_DIR=GOLD, **(env_extra or {}))
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120, env=env)
+
+
+def test_cli_prices_equal_c_abi(ctx):
+    print("gpu cli -p asian all DEMO", flush=True)
+    out = run_cli("-p", "asian", "all", "DEMO", env_extra={"HEXO_PATHS": "20000", "HEXO_STEPS": "200"})
+    assert out.returncode == 0, out.stderr
+    lines = [l for l in out.stdout.splitlines() if l.startswith("S:")]
+    got = [float(re.search(r"asian-option-price: ([-0-9.]+)", l).group(1)) for l in lines]
+    # the chain file: expiries of 20 (dropped, T <= 0.1), 65 and 261 trading days
+    req = H.Request(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0, expiries=[65 / 261, 1.0],
+                    strikes=[[90, 100, 110], [100, 120]], steps=200, n_paths=20000, seed=1, payoffs=H.ASIAN,
+                    precision=H.FP64)
+    p, _ = ctx.price(req)
+    assert len(got) == 5
+    np.testing.assert_allclose(got, np.round(p, 2), atol=0.0051)
+    # volume filter: open interest >= 300 keeps 3 calls
+    out = run_cli("-p", "asian", "all", "DEMO", "open_interest", "300",
+                  env_extra={"HEXO_PATHS": "5000", "HEXO_STEPS": "50"})
+    assert out.returncode == 0 and len([l for l in out.stdout.splitlines() if l.startswith("S:")]) == 3
+
+
+def test_cli_self_tests():
+    print("gpu cli -t rng -t simulation", flush=True)
+    out = run_cli("-t", "rng", "-t", "simulation")
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "RNG test passed" in out.stdout and "Simulation test passed" in out.stdout
+
+
+def test_cli_rejects_bad_grammar():
+    assert run_cli("-p", "european", "all", "DEMO").returncode == 2
+    assert run_cli("-q").returncode == 2

@@ -1,0 +1,168 @@
+"""CPU oracle: the QE scheme, the reference time grid and the Monte-Carlo estimator.
+
+Pins (none of them restates the oracle's formulas):
+ * QE variance step: the exact CIR conditional mean and variance (Andersen 2008, eqs. 17-18) are
+   reproduced by both branches; the exponential branch's mass at 0 is p = (psi-1)/(psi+1).
+ * Log-price step: with rho = 0 the step is conditionally lognormal with E[X'/X] = 1 exactly; in
+   the sigma -> 0 limit with v0 = theta the model is GBM, so European prices equal Black-Scholes.
+ * Time grid (src/HSimulation.tpp:35-44) and the Asian finalisation quirk
+   (src/inc/AsianContract.h:29-34): the case table of n* per (T, N), and the K = 0 invariant
+   E[A] = S0 (N-1)/N when the last step lands on T, S0 otherwise.
+ * Estimator: prices and standard errors equal numpy's mean and std of the per-path payoffs.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.stats import norm
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+BASE = [0.04, 0.04, -0.7, 2.0, 0.3]
+
+
+def cir_moments(V, delta, theta, kappa, eps):
+    e = np.exp(-kappa * delta)
+    m = theta + (V - theta) * e
+    s2 = V * eps * eps * e / kappa * (1 - e) + theta * eps * eps / (2 * kappa) * (1 - e) ** 2
+    return m, s2
+
+
+@pytest.mark.parametrize("V,delta,kappa,theta,eps", [
+    (0.04, 1 / 52, 2.0, 0.04, 0.3),      # quadratic branch (psi small)
+    (0.002, 1 / 252, 0.5, 0.04, 0.9),    # exponential branch (psi large)
+    (0.01, 1 / 12, 1.0, 0.09, 1.0),      # mixed regime
+    (0.0, 1 / 100, 3.0, 0.05, 0.6),      # from V = 0
+])
+def test_qe_step_reproduces_cir_moments(V, delta, kappa, theta, eps):
+    params = [V, theta, -0.5, kappa, eps]
+    n = 2_000_000
+    r = O.qe_moments(params, delta, V, n, seed=11)
+    m, s2 = cir_moments(V, delta, theta, kappa, eps)
+    se_mean = np.sqrt(s2 / n)
+    se_var = np.sqrt(max(r["m4"] - r["var"] ** 2, 1e-300) / n)
+    assert abs(r["mean"] - m) < 5 * se_mean, (r["mean"], m, se_mean)
+    assert abs(r["var"] - s2) < 5 * se_var, (r["var"], s2, se_var)
+    psi = s2 / m ** 2
+    if psi < 1.5:
+        assert r["quad_frac"] == 1.0 and r["zero_frac"] == 0.0
+    else:
+        p = (psi - 1) / (psi + 1)
+        assert r["quad_frac"] == 0.0
+        assert abs(r["zero_frac"] - p) < 5 * np.sqrt(p * (1 - p) / n)
+
+
+def test_log_step_is_martingale_without_correlation():
+    r = O.qe_moments([0.04, 0.04, 0.0, 2.0, 0.5], 1 / 12, 0.04, 2_000_000, seed=3)
+    assert abs(r["mean_exp_dlogx"] - 1.0) < 5 * np.sqrt(0.04 / 12 / 2_000_000) * 1.5
+
+
+def bs_call(S, K, T, vol, r=0.0):
+    d1 = (np.log(S / K) + (r + 0.5 * vol * vol) * T) / (vol * np.sqrt(T))
+    return S * norm.cdf(d1) - K * np.exp(-r * T) * norm.cdf(d1 - vol * np.sqrt(T))
+
+
+def test_black_scholes_limit():
+    """sigma -> 0 with v0 = theta: every step is an exact GBM step with vol sqrt(theta); the
+    rho/eps terms of the log-price step must combine to it (rho = -0.7 keeps them large)."""
+    params = [0.04, 0.04, -0.7, 2.0, 1e-5]
+    K = np.array([80.0, 100.0, 120.0])
+    out = O.price(params, 100.0, [1.0], [3], K, steps=50, n_paths=400_000, seed=21, payoffs=O.EUROPEAN)
+    ref = bs_call(100.0, K, 1.0, 0.2)
+    assert np.all(np.abs(out["prices"] - ref) < 4 * out["std_errs"]), (out["prices"], ref, out["std_errs"])
+    # with the drift and discount extensions the same holds at r = 5%
+    out = O.price(params, 100.0, [1.0], [3], K, steps=50, n_paths=400_000, seed=22, r=0.05,
+                  flags=O.DRIFT | O.DISCOUNT, payoffs=O.EUROPEAN)
+    ref = bs_call(100.0, K, 1.0, 0.2, r=0.05)
+    assert np.all(np.abs(out["prices"] - ref) < 4 * out["std_errs"]), (out["prices"], ref)
+
+
+CASES = [(1.0, 252, 253), (1.0, 2048, 2048), (1.0, 1000, 1000), (0.1, 5, 5), (0.25, 13, 14), (0.5, 26, 27),
+         (1.0, 52, 52), (2.0, 104, 105), (3.0, 156, 157), (4.0, 208, 209), (5.0, 260, 261)]
+
+
+@pytest.mark.parametrize("T,N,nstar", CASES)
+def test_time_grid_case_table(T, N, nstar):
+    k, fs, fw = O.time_grid([T], [1], N)
+    assert k == nstar and int(fs[0]) == nstar
+    if nstar == N:  # case A: the N-th step lands on T; the interpolation weight is ~1
+        assert abs(fw[0] - 1.0) < 1e-9
+    else:           # case B: one more step; weight ~0
+        assert 0.0 <= fw[0] < 1e-9
+
+
+def test_time_grid_matches_product_host_builder():
+    import hestonexotics_amd as H
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        n = int(rng.integers(1, 6))
+        T = np.sort(rng.uniform(0.05, 5.0, n))
+        if np.any(np.diff(T) <= 0):
+            continue
+        ns = rng.integers(0, 4, n)
+        if ns.sum() == 0:
+            ns[-1] = 1
+        N = int(rng.integers(1, 300))
+        a = O.time_grid(T, ns, N)
+        b = H.time_grid(T, ns, N)
+        assert a[0] == b[0]
+        np.testing.assert_array_equal(a[1], b[1])
+        np.testing.assert_array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("T,N,case_a", [(0.1, 5, True), (0.25, 13, False), (1.0, 52, True), (0.5, 26, False)])
+def test_asian_k0_invariant(T, N, case_a):
+    """With K = 0 the Asian payoff is the average itself. Under the GBM limit E[X_t] = S0, so the
+    reference's average has mean S0 (N-1)/N in case A (its last trapezoid is replaced by the
+    price increment X_N - X_{N-1}) and S0 in case B (a proper trapezoid over [0, T])."""
+    params = [0.04, 0.04, -0.7, 2.0, 1e-5]
+    out = O.price(params, 100.0, [T], [1], [0.0], steps=N, n_paths=200_000, seed=9, payoffs=O.ASIAN)
+    expect = 100.0 * (N - 1) / N if case_a else 100.0
+    assert abs(out["prices"][0] - expect) < 5 * out["std_errs"][0], (out["prices"][0], expect)
+    # the corrected-trapezoid extension removes the case A bias
+    out = O.price(params, 100.0, [T], [1], [0.0], steps=N, n_paths=200_000, seed=9, payoffs=O.ASIAN,
+                  flags=O.TRAPEZOID)
+    assert abs(out["prices"][0] - 100.0) < 5 * out["std_errs"][0]
+
+
+def test_estimator_matches_per_path_payoffs():
+    K = np.array([90.0, 100.0, 110.0, 95.0])
+    out = O.price(BASE, 100.0, [0.5, 1.0], [3, 1], K, steps=20, n_paths=5000, seed=4, want_finals=True,
+                  r=0.03, flags=O.DISCOUNT)
+    f = out["finals"]
+    chain = [0, 0, 0, 1]
+    asian = np.stack([np.maximum(f[:, c, 0] - k, 0) for c, k in zip(chain, K)])
+    euro = np.stack([np.maximum(f[:, c, 1] - k, 0) for c, k in zip(chain, K)])
+    pay = np.concatenate([asian, euro])
+    disc = np.exp(-0.03 * np.array([0.5, 0.5, 0.5, 1.0] * 2))
+    np.testing.assert_allclose(out["prices"], disc * pay.mean(axis=1), rtol=1e-12)
+    np.testing.assert_allclose(out["std_errs"], disc * pay.std(axis=1, ddof=1) / np.sqrt(5000), rtol=1e-9)
+
+
+def test_european_mc_agrees_with_closed_form():
+    """The QE scheme (no martingale correction) at 52 steps/yr vs the closed-form Heston price:
+    agreement within 4 standard errors plus a stated 0.5% discretisation allowance."""
+    g = json.load(open(os.path.join(GOLD, "heston_closed_form.json")))
+    case = [c for c in g["cases"] if c["S"] == 100.0 and c["T"] == 1.0][0]
+    K = np.array(case["K"])
+    params = [case["v0"], case["theta"], case["rho"], case["kappa"], case["sigma"]]
+    out = O.price(params, 100.0, [1.0], [len(K)], K, steps=52, n_paths=300_000, seed=77, r=case["r"],
+                  flags=O.DRIFT | O.DISCOUNT, payoffs=O.EUROPEAN)
+    ref = np.array(case["price"])
+    tol = 4 * out["std_errs"] + 0.005 * ref
+    assert np.all(np.abs(out["prices"] - ref) < tol), (out["prices"], ref, out["std_errs"])
+
+
+def test_results_independent_of_thread_count():
+    a = O.price(BASE, 100.0, [1.0], [2], [95.0, 105.0], steps=30, n_paths=20_000, seed=1, n_threads=1)
+    b = O.price(BASE, 100.0, [1.0], [2], [95.0, 105.0], steps=30, n_paths=20_000, seed=1, n_threads=4)
+    np.testing.assert_array_equal(a["prices"], b["prices"])
+
+
+def test_path_offset_selects_the_same_paths():
+    full = O.price(BASE, 100.0, [1.0], [1], [100.0], steps=12, n_paths=300, seed=8, want_finals=True)["finals"]
+    part = O.price(BASE, 100.0, [1.0], [1], [100.0], steps=12, n_paths=100, seed=8, path_offset=150,
+                   want_finals=True)["finals"]
+    np.testing.assert_array_equal(full[150:250], part)
