@@ -31,8 +31,9 @@ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
         const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        // three-input XOR in one v_bitop3_b32 (truth table 0x96)
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
         c0 = n0;
         c1 = (uint32_t)p1;
         c2 = n2;
@@ -137,6 +138,28 @@ __device__ __forceinline__ float normal_f32(uint32_t k) {
     const float num = central ? nc : nt;
     const float den = central ? dc : dt;
     return num * __builtin_amdgcn_rcpf(den);
+}
+
+// The two halves of normal_f32, for kernels that evaluate the tail region only where needed.
+// Central region: returns q A(r)/B(r) and sets `central` = |q| <= 0.425 (the value is meaningless
+// when `central` is false).
+__device__ __forceinline__ float normal_central_f32(uint32_t k, bool& central) {
+    const int32_t sk = (int32_t)(k ^ 0x80000000u);
+    const float q = __builtin_fmaf((float)sk, 0x1p-32f, 0x1p-33f);
+    const float rc = __builtin_fmaf(-q, q, 0.180625f);
+    const float nc = q * horner8f(As241<float>::a, rc);
+    const float dc = horner8f(As241<float>::b, rc);
+    central = __builtin_fabsf(q) <= 0.425f;
+    return nc * __builtin_amdgcn_rcpf(dc);
+}
+// Tail region (|q| > 0.425): r = sqrt(-ln(min(u, 1-u))) - 1.6, C(r)/D(r) with the sign of q.
+__device__ __forceinline__ float normal_tail_f32(uint32_t k) {
+    const int32_t sk = (int32_t)(k ^ 0x80000000u);
+    const uint32_t mn = k ^ ~(uint32_t)(sk >> 31);
+    const float rr = __builtin_fmaf((float)mn, 0x1p-32f, 0x1p-33f);
+    const float rt = __builtin_amdgcn_sqrtf(-0.693147180559945309f * __builtin_amdgcn_logf(rr)) - 1.6f;
+    const float z = horner8f(As241<float>::c, rt) * __builtin_amdgcn_rcpf(horner8f(As241<float>::d, rt));
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, z) ^ ((uint32_t)sk & 0x80000000u));
 }
 
 __device__ __forceinline__ bool normal_is_central(uint32_t k) {

@@ -106,6 +106,7 @@ __device__ __forceinline__ void reduce_tiles(const PathArgs& a, const DevReq& R,
             const uint64_t p0 = (uint64_t)tile * kTile;
             const uint64_t n_valid64 = R.n_paths - p0;
             const int n_valid = n_valid64 >= (uint64_t)kTile ? kTile : (int)n_valid64;
+            (void)n_valid;
             const size_t slot = (size_t)R.partial_base + (size_t)tile * n_out + o;
             if constexpr (sizeof(ValT) == 8) {
                 // FP32 kernel: payoffs in FP32, sums in FP64 or compensated FP32
@@ -116,8 +117,8 @@ __device__ __forceinline__ void reduce_tiles(const PathArgs& a, const DevReq& R,
                     for (int i = 0; i < 8; ++i) {
                         const int p = g * 8 + i;
                         const float2 v = reinterpret_cast<const float2*>(vals)[j * kTile + p];
-                        const float x = __builtin_fmaxf((use_xt ? v.y : v.x) - K, 0.0f);
-                        const double xd = p < n_valid ? (double)x : 0.0;
+                        // paths beyond n_paths hold -inf, so their payoff is 0
+                        const double xd = (double)__builtin_fmaxf((use_xt ? v.y : v.x) - K, 0.0f);
                         s += xd;
                         ss = __builtin_fma(xd, xd, ss);
                     }
@@ -133,8 +134,7 @@ __device__ __forceinline__ void reduce_tiles(const PathArgs& a, const DevReq& R,
                     for (int i = 0; i < 8; ++i) {
                         const int p = g * 8 + i;
                         const float2 v = reinterpret_cast<const float2*>(vals)[j * kTile + p];
-                        float x = __builtin_fmaxf((use_xt ? v.y : v.x) - K, 0.0f);
-                        x = p < n_valid ? x : 0.0f;
+                        const float x = __builtin_fmaxf((use_xt ? v.y : v.x) - K, 0.0f);
                         s.add(x);
                         ss.add(x * x);
                     }
@@ -197,55 +197,144 @@ struct SegF {
     float d, mth, c1, c2, k0, kc, kd, k3;
 };
 
-// One QE step in FP32 (Andersen's QE scheme as in src/HSimulation.tpp:52-86): variance by the
-// quadratic branch a (b + Z_V)^2 when psi = s2/m^2 < 1.5, else the exponential branch; then the
-// central log-price step. Y is log2(X/S0); X = 2^Y.
-__device__ __forceinline__ bool qe_step_fp32(float& V, float& Y, float& X, float& Xp, uint32_t wv, uint32_t wx,
-                                             const SegF& c) {
-    const float m = __builtin_fmaf(V, c.d, c.mth);
-    const float s2 = __builtin_fmaf(V, c.c1, c.c2);
-    const float m2 = m * m;
-    const bool quad = s2 < 1.5f * m2;
-    float Vn;
-    if (quad) {
-        const float inv = __builtin_amdgcn_rcpf(s2);
-        const float t = (m2 + m2) * inv;  // 2 / psi
-        const float tm1 = t - 1.0f;
-        const float b2 = tm1 + __builtin_amdgcn_sqrtf(t * tm1);
-        const float b = __builtin_amdgcn_sqrtf(b2);
-        const float aq = m * __builtin_amdgcn_rcpf(1.0f + b2);
-        const float bz = b + normal_f32(wv);
-        Vn = aq * bz * bz;
-    } else {
-        // p = (psi-1)/(psi+1), beta = 2/(m (psi+1)); V' = ln((1-p)/(1-U))/beta if U > p, else 0.
-        // With sm = s2 + m^2: 1-p = 2m^2/sm and 1/beta = sm/(2m).
-        const float sm = s2 + m2;
-        const float lhs = one_minus_uniform_f32(wv) * sm;  // (1-U) sm
-        const float tm2 = m2 + m2;
-        const float lg = __builtin_amdgcn_logf(tm2 * __builtin_amdgcn_rcpf(lhs));
-        const float v = (sm * __builtin_amdgcn_rcpf(m)) * (0.5f * 0.693147180559945309f) * lg;
-        Vn = lhs < tm2 ? v : 0.0f;
+// Per-wave LDS scratch of the FP32 kernel: the tail-normal queue during the steps (2*PPT words in,
+// 2*PPT results out per lane) and the (A, X_T) transpose at expiries, which reuses the same space.
+template <int PPT>
+struct WaveLds {
+    static constexpr int kWords = 2 * (2 * PPT * kTile);  // queue in + queue out, 32-bit words
+};
+
+// AS241 tail evaluations gathered across the wave. Each lane holds N normals whose central-region
+// value is already in z[s]; where t[s] is set (|q| > 0.425) the tail rational is needed instead.
+// The wave packs those words into an LDS queue (ballot + mbcnt give each one a slot), evaluates
+// the queue densely 64 entries at a time, and each lane reads its results back. With 15% tail
+// draws this replaces N full-wave tail evaluations by ceil(0.15 * 64 N / 64) dense ones.
+template <int N>
+__device__ __forceinline__ void tail_normals(const uint32_t (&w)[N], const bool (&t)[N], float (&z)[N], uint32_t* q,
+                                             int lane) {
+    uint32_t* q_in = q;
+    float* q_out = reinterpret_cast<float*>(q + N * kTile);
+    uint32_t slot[N];
+    uint32_t total = 0;  // wave-uniform
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+        const uint64_t mask = __ballot(t[s] ? 1 : 0);
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        slot[s] = total + rank;
+        if (t[s]) q_in[slot[s]] = w[s];
+        total += (uint32_t)__builtin_popcountll(mask);
     }
-    const float sum = V + Vn;
-    const float dif = Vn - V;
-    const float inc = __builtin_fmaf(c.kd, dif, __builtin_fmaf(c.kc, sum, c.k0));
-    const float sq = __builtin_amdgcn_sqrtf(c.k3 * sum);
-    Y = __builtin_fmaf(sq, normal_f32(wx), Y + inc);
-    Xp = X;
-    X = __builtin_amdgcn_exp2f(Y);
-    V = Vn;
-    return quad;
+    total = uniform(total);
+    wave_sync_lds();
+    for (uint32_t p = 0; p < total; p += kTile) {
+        const uint32_t i = p + (uint32_t)lane;
+        if (i < total) q_out[i] = normal_tail_f32(q_in[i]);
+    }
+    wave_sync_lds();
+#pragma unroll
+    for (int s = 0; s < N; ++s)
+        if (t[s]) z[s] = q_out[slot[s]];
+    wave_sync_lds();  // the queue is rewritten by the next step
+}
+
+struct StepCount {
+    uint32_t quad = 0, exp = 0, central = 0, tail = 0;
+};
+
+// One QE step for the PPT paths of a lane (Andersen's QE scheme as in src/HSimulation.tpp:52-86):
+// variance by the quadratic branch a (b + Z_V)^2 when psi = s2/m^2 < 1.5, else the exponential
+// branch; then the central log-price step. Y is log2(X/S0); X = 2^Y; P accumulates X for the
+// Asian sum. Z_V (quadratic lanes) and Z_X come from AS241 with the tails gathered per wave.
+template <int PPT, bool COUNT>
+__device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], float (&X)[PPT], float (&Xp)[PPT],
+                                             float (&P)[PPT], const uint32_t (&wv)[PPT], const uint32_t (&wx)[PPT],
+                                             const SegF& c, uint32_t* q, int lane, const bool (&valid)[PPT],
+                                             StepCount& cnt) {
+    float m[PPT], s2[PPT], m2[PPT];
+    bool quad[PPT];
+    uint32_t w[2 * PPT];
+    bool t[2 * PPT];
+    float z[2 * PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        m[j] = __builtin_fmaf(V[j], c.d, c.mth);
+        s2[j] = __builtin_fmaf(V[j], c.c1, c.c2);
+        m2[j] = m[j] * m[j];
+        quad[j] = s2[j] < 1.5f * m2[j];
+        bool cv, cx;
+        z[2 * j] = normal_central_f32(wv[j], cv);
+        z[2 * j + 1] = normal_central_f32(wx[j], cx);
+        w[2 * j] = wv[j];
+        w[2 * j + 1] = wx[j];
+        t[2 * j] = quad[j] && !cv;
+        t[2 * j + 1] = !cx;
+        if constexpr (COUNT) {
+            if (valid[j]) {
+                cnt.quad += quad[j] ? 1u : 0u;
+                cnt.exp += quad[j] ? 0u : 1u;
+                cnt.central += (quad[j] && cv ? 1u : 0u) + (cx ? 1u : 0u);
+                cnt.tail += (t[2 * j] ? 1u : 0u) + (t[2 * j + 1] ? 1u : 0u);
+            }
+        }
+    }
+    tail_normals<2 * PPT>(w, t, z, q, lane);
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        float Vn;
+        if (quad[j]) {
+            const float inv = __builtin_amdgcn_rcpf(s2[j]);
+            const float tt = (m2[j] + m2[j]) * inv;  // 2 / psi
+            const float tm1 = tt - 1.0f;
+            const float b2 = tm1 + __builtin_amdgcn_sqrtf(tt * tm1);
+            const float b = __builtin_amdgcn_sqrtf(b2);
+            const float aq = m[j] * __builtin_amdgcn_rcpf(1.0f + b2);
+            const float bz = b + z[2 * j];
+            Vn = aq * bz * bz;
+        } else {
+            // p = (psi-1)/(psi+1), beta = 2/(m (psi+1)); V' = ln((1-p)/(1-U))/beta if U > p, else 0.
+            // With sm = s2 + m^2: 1-p = 2m^2/sm and 1/beta = sm/(2m).
+            const float sm = s2[j] + m2[j];
+            const float lhs = one_minus_uniform_f32(wv[j]) * sm;  // (1-U) sm
+            const float tm2 = m2[j] + m2[j];
+            const float lg = __builtin_amdgcn_logf(tm2 * __builtin_amdgcn_rcpf(lhs));
+            const float v = (sm * __builtin_amdgcn_rcpf(m[j])) * (0.5f * 0.693147180559945309f) * lg;
+            Vn = lhs < tm2 ? v : 0.0f;
+        }
+        const float sum = V[j] + Vn;
+        const float dif = Vn - V[j];
+        const float inc = __builtin_fmaf(c.kd, dif, __builtin_fmaf(c.kc, sum, c.k0));
+        const float sq = __builtin_amdgcn_sqrtf(c.k3 * sum);
+        Y[j] = __builtin_fmaf(sq, z[2 * j + 1], Y[j] + inc);
+        Xp[j] = X[j];
+        X[j] = __builtin_amdgcn_exp2f(Y[j]);
+        P[j] += X[j];
+        V[j] = Vn;
+    }
+}
+
+template <int PPT>
+__device__ __forceinline__ void philox_block(uint32_t blk, const uint32_t (&plo)[PPT], const uint32_t (&phi)[PPT],
+                                             uint32_t key0, uint32_t key1, uint32_t (&w0)[PPT], uint32_t (&w1)[PPT],
+                                             uint32_t (&w2)[PPT], uint32_t (&w3)[PPT]) {
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const u32x4 r = philox4x32_10(blk, plo[j], phi[j], 0u, key0, key1);
+        w0[j] = r.x;
+        w1[j] = r.y;
+        w2[j] = r.z;
+        w3[j] = r.w;
+    }
 }
 
 template <int PPT, bool DEBUG, bool COUNT>
-__device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32_t tile0, int lane, float2* vals) {
+__device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32_t tile0, int lane, uint32_t* lds) {
     const DevReq& R = a.reqs[ri];
     const uint64_t n_paths = R.n_paths;
     const uint32_t key0 = R.seed_lo, key1 = R.seed_hi;
     float V[PPT], Y[PPT], X[PPT], Xp[PPT], P[PPT], acc[PPT], Xs[PPT];
-    uint32_t plo[PPT], phi[PPT], wb0[PPT], wb1[PPT];
+    uint32_t plo[PPT], phi[PPT], sv[PPT], sx[PPT];
     bool valid[PPT];
-    uint32_t cq = 0, ce = 0, cc = 0, ct = 0;
+    StepCount cnt;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         const uint64_t pl = (uint64_t)(tile0 + (uint32_t)j) * kTile + (uint64_t)lane;
@@ -260,53 +349,38 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
         P[j] = 0.0f;
         acc[j] = 0.0f;
         Xs[j] = 1.0f;
-        wb0[j] = wb1[j] = 0u;
+        sv[j] = sx[j] = 0u;
     }
-    uint32_t k = 0;
+    uint32_t k = 0;  // global step index; step k uses words 2k, 2k+1 of the path's stream
     const uint32_t n_segs = R.n_segs;
     for (uint32_t s = 0; s < n_segs; ++s) {
         const DevSeg& S = a.segs[R.seg_begin + s];
         const SegF c{S.d, S.mth, S.c1, S.c2, S.k0, S.kc, S.kd, S.k3};
         const uint32_t n = S.n_steps;
-        for (uint32_t i = 0; i < n; ++i, ++k) {
-            uint32_t wv[PPT], wx[PPT];
-            if ((k & 1u) == 0u) {
-#pragma unroll
-                for (int j = 0; j < PPT; ++j) {
-                    const u32x4 w = philox4x32_10(k >> 1, plo[j], phi[j], 0u, key0, key1);
-                    wv[j] = w.x;
-                    wx[j] = w.y;
-                    wb0[j] = w.z;
-                    wb1[j] = w.w;
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < PPT; ++j) {
-                    wv[j] = wb0[j];
-                    wx[j] = wb1[j];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < PPT; ++j) {
-                const bool quad = qe_step_fp32(V[j], Y[j], X[j], Xp[j], wv[j], wx[j], c);
-                if constexpr (COUNT) {
-                    if (valid[j]) {
-                        cq += quad ? 1u : 0u;
-                        ce += quad ? 0u : 1u;
-                        const bool cv = normal_is_central(wv[j]), cx = normal_is_central(wx[j]);
-                        cc += (quad && cv ? 1u : 0u) + (cx ? 1u : 0u);
-                        ct += (quad && !cv ? 1u : 0u) + (cx ? 0u : 1u);
-                    }
-                }
-            }
-            if (i + 1u < n) {
-#pragma unroll
-                for (int j = 0; j < PPT; ++j) P[j] += X[j];
-            }
+        uint32_t i = 0;
+        if ((k & 1u) && n > 0) {  // second half of a block opened by the previous run
+            qe_step_fp32<PPT, COUNT>(V, Y, X, Xp, P, sv, sx, c, lds, lane, valid, cnt);
+            ++i;
+            ++k;
         }
-        // Asian accumulation of the run's interior steps: delta * (X_{b-1}/2 + X_b + ... + X_{e-1}/2)
+        for (; i + 2 <= n; i += 2, k += 2) {
+            uint32_t w0[PPT], w1[PPT], w2[PPT], w3[PPT];
+            philox_block<PPT>(k >> 1, plo, phi, key0, key1, w0, w1, w2, w3);
+            qe_step_fp32<PPT, COUNT>(V, Y, X, Xp, P, w0, w1, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT>(V, Y, X, Xp, P, w2, w3, c, lds, lane, valid, cnt);
+        }
+        if (i < n) {
+            uint32_t w0[PPT], w1[PPT];
+            philox_block<PPT>(k >> 1, plo, phi, key0, key1, w0, w1, sv, sx);
+            qe_step_fp32<PPT, COUNT>(V, Y, X, Xp, P, w0, w1, c, lds, lane, valid, cnt);
+            ++k;
+        }
+        // Asian accumulation of the run's interior steps b..e-1:
+        // delta * (X_{b-1}/2 + X_b + ... + X_{e-2} + X_{e-1}/2) = delta * (P - X_e + (Xs - Xp)/2)
 #pragma unroll
-        for (int j = 0; j < PPT; ++j) acc[j] = __builtin_fmaf(S.delta_f, P[j] + 0.5f * (Xs[j] - Xp[j]), acc[j]);
+        for (int j = 0; j < PPT; ++j)
+            acc[j] = __builtin_fmaf(S.delta_f, (P[j] - X[j]) + 0.5f * (Xs[j] - Xp[j]), acc[j]);
+        float2* vals = reinterpret_cast<float2*>(lds);
         for (uint32_t f = 0; f < S.fin_count; ++f) {
             const DevFin& F = a.fins[S.fin_begin + f];
             const float w = F.w_f, invT = F.invT_f;
@@ -316,7 +390,8 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
                 const float xt = __builtin_fmaf(dX, w, Xp[j]);
                 const float A = (R.flags & kTrapezoid) ? (acc[j] + F.hwdt_f * (Xp[j] + xt)) * invT
                                                        : __builtin_fmaf(dX, w, acc[j]) * invT;
-                vals[j * kTile + lane] = make_float2(A, xt);
+                // paths beyond n_paths carry -inf: every payoff max(0, . - K) is then 0
+                vals[j * kTile + lane] = valid[j] ? make_float2(A, xt) : make_float2(-INFINITY, -INFINITY);
                 if constexpr (DEBUG) {
                     if (valid[j] && a.finals) {
                         const uint64_t pl = (uint64_t)(tile0 + (uint32_t)j) * kTile + (uint64_t)lane;
@@ -338,24 +413,23 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
             P[j] = 0.0f;
         }
     }
-    if constexpr (COUNT) count_flush(a.counts, cq, ce, cc, ct, lane);
+    if constexpr (COUNT) count_flush(a.counts, cnt.quad, cnt.exp, cnt.central, cnt.tail, lane);
 }
 
 template <int PPT, bool DEBUG, bool COUNT>
 __global__ __launch_bounds__(256) void k_path_fp32(PathArgs a) {
-    extern __shared__ float2 lds_f2[];
+    extern __shared__ uint32_t lds_u32[];
     const int lane = (int)(threadIdx.x & 63u);
-    float2* vals = lds_f2 + (threadIdx.x >> 6) * (PPT * kTile);
+    uint32_t* lds = lds_u32 + (threadIdx.x >> 6) * WaveLds<PPT>::kWords;
     for (;;) {
         const uint32_t item = next_item(a.counter, lane);
         if (item >= a.total_items) break;
         const uint32_t sp = uniform(find_sorted(a.item_end, a.n_sorted, item));
         const uint32_t first = sp ? a.item_end[sp - 1] : 0u;
         const uint32_t ri = uniform(a.order[sp]);
-        path_fp32<PPT, DEBUG, COUNT>(a, ri, uniform((item - first) * (uint32_t)PPT), lane, vals);
+        path_fp32<PPT, DEBUG, COUNT>(a, ri, uniform((item - first) * (uint32_t)PPT), lane, lds);
     }
 }
-
 // ------------------------------------------------------------------------------------------
 // Plain FP64 path kernel: the reference's step and policies in FP64, recomputing the step's
 // moments and K0..K4 every step exactly as written in src/HSimulation.tpp:52-86.
@@ -640,7 +714,8 @@ const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode) 
 
 size_t path_lds_bytes(uint32_t precision, uint32_t ppt, uint32_t block) {
     const size_t waves = block / kTile;
-    return precision == kPrecFP64 ? waves * kTile * sizeof(double2) : waves * ppt * kTile * sizeof(float2);
+    // FP32: per wave the tail-normal queue (2 * 2 * ppt * 64 words), reused by the (A, X_T) transpose
+    return precision == kPrecFP64 ? waves * kTile * sizeof(double2) : waves * 4 * ppt * kTile * sizeof(uint32_t);
 }
 
 hipError_t launch_paths(uint32_t precision, uint32_t ppt, PathMode mode, uint32_t grid, uint32_t block,
