@@ -124,6 +124,8 @@ def main():
                     help="profiling driver: 8 eager FP32 passes of the workload, then a plain copy of "
                          "--copy-mib MiB (the known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--copy-mib", type=int, default=512)
+    ap.add_argument("--sweep", action="store_true",
+                    help="time the FP32 workload under every launch shape the autotuner considers, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by paths or "
@@ -145,6 +147,18 @@ def main():
         bp.close()
         ms = ctx.copy_probe(args.copy_mib << 20, reps=3)
         log(f"copy probe: {args.copy_mib} MiB in {ms:.3f} ms ({(2 * (args.copy_mib << 20)) / ms / 1e6:.1f} GB/s)")
+        ctx.close()
+        return
+    if args.sweep:
+        reqs = W.c2(n_paths=args.paths, precision=H.FP32)
+        for ppt in (1, 2, 4):
+            for blk in (64, 128, 256):
+                for wps in (0, 2, 4, 8):
+                    ctx.set_tuning(ppt, blk, wps)
+                    b = ctx.batch(reqs)
+                    _, kms = time_batch(ctx, b, 5, 1)
+                    b.close()
+                    log(f"sweep ppt={ppt} block={blk} waves_per_simd={wps}: path kernel {kms:.3f} ms")
         ctx.close()
         return
     reqs32 = W.c2(n_paths=args.paths, precision=H.FP32)

@@ -283,10 +283,13 @@ int hexo_paths(hexo_ctx* ctx, const hexo_request* req, uint64_t first, uint64_t 
     if (n > (1ull << 26)) return ctx->set_error(HEXO_ERR_INVALID, "at most 2^26 paths per hexo_paths call");
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     DeviceGuard dg(ctx->device);
+    // simulate from the tile boundary at or below `first` (path offsets are tile-aligned) and
+    // return the requested window
+    const uint64_t lead = first % hexo::kTile;
     hexo_request sub = *req;
-    sub.path_offset = req->path_offset + first;
-    sub.n_paths = n;
-    const uint64_t n_fin = n * (uint64_t)req->n_expiries * 2;
+    sub.path_offset = req->path_offset + (first - lead);
+    sub.n_paths = n + lead;
+    const uint64_t n_fin = sub.n_paths * (uint64_t)req->n_expiries * 2;
     std::string err;
     std::unique_ptr<hexo::Batch> b;
     int rc = hexo::batch_create(ctx, &sub, 1, ctx->stream, nullptr, b, err, &n_fin);
@@ -299,7 +302,8 @@ int hexo_paths(hexo_ctx* ctx, const hexo_request* req, uint64_t first, uint64_t 
     rc = hexo::batch_enqueue(*b, PathMode::Debug, false, d_fin, err);
     if (rc == HEXO_OK) rc = hexo::batch_finish(*b, err);
     if (rc != HEXO_OK) return ctx->set_error(rc, err);
-    if (hipMemcpy(finals, d_fin, sizeof(double) * n_fin, hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(finals, d_fin + lead * (uint64_t)req->n_expiries * 2, sizeof(double) * n * req->n_expiries * 2,
+                  hipMemcpyDeviceToHost) != hipSuccess)
         return ctx->set_error(HEXO_ERR_HIP, "copy of finals failed");
     return HEXO_OK;
 }

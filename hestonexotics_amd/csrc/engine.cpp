@@ -96,6 +96,7 @@ std::string validate(const hexo_request* r) {
     if (r->n_paths == 0) return "n_paths must be >= 1";
     if (r->n_paths > (1ull << 38)) return "n_paths must be <= 2^38";
     if (r->path_offset > ~0ull - r->n_paths) return "path_offset + n_paths overflows";
+    if (r->path_offset % kTile) return "path_offset must be a multiple of 64 (one tile)";
     if (r->steps == 0 || r->steps > (1u << 22)) return "steps must lie in [1, 2^22]";
     if (r->n_expiries == 0 || r->n_expiries > 4096) return "n_expiries must lie in [1, 4096]";
     if (!r->expiries || !r->n_strikes) return "expiries and n_strikes must be non-NULL";

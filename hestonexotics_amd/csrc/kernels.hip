@@ -78,99 +78,128 @@ struct CompF {
 };
 
 // Per-expiry tile reduction. vals[j*64 + p] holds (A, X_T) of path p of sub-tile j (in the
-// kernel's units). Lane (o = lane/8, g = lane%8) sums the payoffs of paths 8g..8g+7 for output o
-// in path order; the 8 group sums are combined by a fixed xor tree. Output o of the request is
-// payoff-major: o = pay * n_opts + option.
-template <int PPT, typename ValT>
-__device__ __forceinline__ void reduce_tiles(const PathArgs& a, const DevReq& R, const DevFin& F, uint32_t tile0,
-                                             int lane, const ValT* vals, bool fp32_units) {
+// kernel's units); paths beyond n_paths hold values whose payoff is 0 (FP32: -inf; FP64: masked).
+// Output o of the request is payoff-major: o = pay * n_opts + option. The summation order of a
+// tile's 64 payoffs depends only on the number of strikes of the expiry (a property of the
+// request), never on the launch shape:
+//  * many strikes (>= kWideStrikes): lane i takes strike i and sums the 64 paths in path order,
+//    reading each path's values as an LDS broadcast;
+//  * few strikes: lane (o = lane/8, g = lane%8) sums paths 8g..8g+7 of output o in path order and
+//    the 8 group sums are combined by a fixed xor tree.
+constexpr uint32_t kWideStrikes = 48;
+
+template <typename Acc>
+struct TileSums;
+
+template <>
+struct TileSums<double> {  // FP64 sums (HEXO_FP32 and the FP64 kernel)
+    double s = 0.0, ss = 0.0;
+    __device__ __forceinline__ void add(double x) {
+        s += x;
+        ss = __builtin_fma(x, x, ss);
+    }
+    __device__ __forceinline__ void tree8() {
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1) {
+            s += __shfl_xor(s, m);
+            ss += __shfl_xor(ss, m);
+        }
+    }
+    __device__ __forceinline__ void store(void* partials, size_t slot) const {
+        reinterpret_cast<double2*>(partials)[slot] = make_double2(s, ss);
+    }
+};
+
+template <>
+struct TileSums<float> {  // compensated FP32 sums (HEXO_FP32_COMPENSATED)
+    CompF s{0.f, 0.f}, ss{0.f, 0.f};
+    __device__ __forceinline__ void add(float x) {
+        s.add(x);
+        ss.add(x * x);
+    }
+    __device__ __forceinline__ void tree8() {
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1) {
+            const float s2 = __shfl_xor(s.s, m), c2 = __shfl_xor(s.c, m);
+            const float q2 = __shfl_xor(ss.s, m), d2 = __shfl_xor(ss.c, m);
+            s.merge(s2, c2);
+            ss.merge(q2, d2);
+        }
+    }
+    __device__ __forceinline__ void store(void* partials, size_t slot) const {
+        reinterpret_cast<float4*>(partials)[slot] = make_float4(s.s, s.c, ss.s, ss.c);
+    }
+};
+
+// payoff of one path value for strike K; `v` is the FP32 kernel's float2 or the FP64 kernel's double2
+__device__ __forceinline__ float payoff_of(float2 v, bool use_xt, float K, bool) {
+    return __builtin_fmaxf((use_xt ? v.y : v.x) - K, 0.0f);
+}
+__device__ __forceinline__ double payoff_of(double2 v, bool use_xt, double K, bool valid) {
+    const double x = (use_xt ? v.y : v.x) - K;
+    return (valid && x > 0.0) ? x : 0.0;
+}
+
+template <int PPT, typename ValT, typename Acc>
+__device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevReq& R, const DevFin& F,
+                                                  uint32_t tile0, int lane, const ValT* vals) {
+    using KT = decltype(ValT{}.x);
     const uint32_t nk = F.n_strikes;
-    const uint32_t npay = R.n_pay;
-    const uint32_t n_out_c = nk * npay;
-    const uint32_t n_out = R.n_opts * npay;
-    const int g = lane & 7;
-    const bool comp = R.precision == kPrecComp;
-    for (uint32_t base = 0; base < n_out_c; base += 8) {
-        const uint32_t ol = base + (uint32_t)(lane >> 3);
-        const bool active = ol < n_out_c;
-        const uint32_t olc = active ? ol : 0u;
-        const uint32_t pay = olc / nk;
-        const uint32_t kj = olc - pay * nk;
-        const bool use_xt = (pay == 1u) || !(R.payoffs & kAsian);
-        const DevOpt& opt = a.opts[R.opt_begin + F.opt_begin + kj];
-        const uint32_t o = pay * R.n_opts + F.opt_begin + kj;
+    const uint32_t n_out = R.n_opts * R.n_pay;
+    const bool asian = (R.payoffs & kAsian) != 0;
+    for (uint32_t pay = 0; pay < R.n_pay; ++pay) {
+        const bool use_xt = pay == 1u || !asian;  // wave-uniform
+        if (nk >= kWideStrikes) {
+            for (uint32_t kb = 0; kb < nk; kb += kTile) {
+                const uint32_t kj = kb + (uint32_t)lane;
+                const bool active = kj < nk;
+                const DevOpt& opt = a.opts[R.opt_begin + F.opt_begin + (active ? kj : 0u)];
+                const KT K = sizeof(KT) == 4 ? (KT)opt.k_rel : (KT)opt.k_abs;
+                const uint32_t o = pay * R.n_opts + F.opt_begin + kj;
 #pragma unroll
-        for (int j = 0; j < PPT; ++j) {
-            const uint32_t tile = tile0 + (uint32_t)j;
-            if (tile >= R.n_tiles) break;
-            const uint64_t p0 = (uint64_t)tile * kTile;
-            const uint64_t n_valid64 = R.n_paths - p0;
-            const int n_valid = n_valid64 >= (uint64_t)kTile ? kTile : (int)n_valid64;
-            (void)n_valid;
-            const size_t slot = (size_t)R.partial_base + (size_t)tile * n_out + o;
-            if constexpr (sizeof(ValT) == 8) {
-                // FP32 kernel: payoffs in FP32, sums in FP64 or compensated FP32
-                const float K = opt.k_rel;
-                if (!comp) {
-                    double s = 0.0, ss = 0.0;
+                for (int j = 0; j < PPT; ++j) {
+                    const uint32_t tile = tile0 + (uint32_t)j;
+                    if (tile >= R.n_tiles) break;
+                    const uint64_t nv = R.n_paths - (uint64_t)tile * kTile;
+                    TileSums<Acc> t;
+                    for (int p = 0; p < kTile; ++p) t.add((Acc)payoff_of(vals[j * kTile + p], use_xt, K, (uint64_t)p < nv));
+                    if (active) t.store(a.partials, (size_t)R.partial_base + (size_t)tile * n_out + o);
+                }
+            }
+        } else {
+            const int g = lane & 7;
+            for (uint32_t kb = 0; kb < nk; kb += 8) {
+                const uint32_t kj = kb + (uint32_t)(lane >> 3);
+                const bool active = kj < nk;
+                const DevOpt& opt = a.opts[R.opt_begin + F.opt_begin + (active ? kj : 0u)];
+                const KT K = sizeof(KT) == 4 ? (KT)opt.k_rel : (KT)opt.k_abs;
+                const uint32_t o = pay * R.n_opts + F.opt_begin + kj;
+#pragma unroll
+                for (int j = 0; j < PPT; ++j) {
+                    const uint32_t tile = tile0 + (uint32_t)j;
+                    if (tile >= R.n_tiles) break;
+                    const uint64_t nv = R.n_paths - (uint64_t)tile * kTile;
+                    TileSums<Acc> t;
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
                         const int p = g * 8 + i;
-                        const float2 v = reinterpret_cast<const float2*>(vals)[j * kTile + p];
-                        // paths beyond n_paths hold -inf, so their payoff is 0
-                        const double xd = (double)__builtin_fmaxf((use_xt ? v.y : v.x) - K, 0.0f);
-                        s += xd;
-                        ss = __builtin_fma(xd, xd, ss);
+                        t.add((Acc)payoff_of(vals[j * kTile + p], use_xt, K, (uint64_t)p < nv));
                     }
-#pragma unroll
-                    for (int m = 1; m < 8; m <<= 1) {
-                        s += __shfl_xor(s, m);
-                        ss += __shfl_xor(ss, m);
-                    }
-                    if (g == 0 && active) reinterpret_cast<double2*>(a.partials)[slot] = make_double2(s, ss);
-                } else {
-                    CompF s{0.f, 0.f}, ss{0.f, 0.f};
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const int p = g * 8 + i;
-                        const float2 v = reinterpret_cast<const float2*>(vals)[j * kTile + p];
-                        const float x = __builtin_fmaxf((use_xt ? v.y : v.x) - K, 0.0f);
-                        s.add(x);
-                        ss.add(x * x);
-                    }
-#pragma unroll
-                    for (int m = 1; m < 8; m <<= 1) {
-                        const float s2 = __shfl_xor(s.s, m), c2 = __shfl_xor(s.c, m);
-                        const float q2 = __shfl_xor(ss.s, m), d2 = __shfl_xor(ss.c, m);
-                        s.merge(s2, c2);
-                        ss.merge(q2, d2);
-                    }
-                    if (g == 0 && active) reinterpret_cast<float4*>(a.partials)[slot] = make_float4(s.s, s.c, ss.s, ss.c);
+                    t.tree8();
+                    if (g == 0 && active) t.store(a.partials, (size_t)R.partial_base + (size_t)tile * n_out + o);
                 }
-            } else {
-                // FP64 kernel
-                const double K = opt.k_abs;
-                double s = 0.0, ss = 0.0;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int p = g * 8 + i;
-                    const double2 v = reinterpret_cast<const double2*>(vals)[j * kTile + p];
-                    double x = (use_xt ? v.y : v.x) - K;
-                    x = x > 0.0 ? x : 0.0;
-                    const double xd = p < n_valid ? x : 0.0;
-                    s += xd;
-                    ss += xd * xd;
-                }
-#pragma unroll
-                for (int m = 1; m < 8; m <<= 1) {
-                    s += __shfl_xor(s, m);
-                    ss += __shfl_xor(ss, m);
-                }
-                if (g == 0 && active) reinterpret_cast<double2*>(a.partials)[slot] = make_double2(s, ss);
             }
         }
     }
-    (void)fp32_units;
+}
+
+template <int PPT, typename ValT>
+__device__ __forceinline__ void reduce_tiles(const PathArgs& a, const DevReq& R, const DevFin& F, uint32_t tile0,
+                                             int lane, const ValT* vals, bool) {
+    if (sizeof(ValT) == sizeof(float2) && R.precision == kPrecComp)
+        reduce_tiles_impl<PPT, ValT, float>(a, R, F, tile0, lane, vals);
+    else
+        reduce_tiles_impl<PPT, ValT, double>(a, R, F, tile0, lane, vals);
 }
 
 __device__ __forceinline__ void count_flush(unsigned long long* counts, uint32_t c0, uint32_t c1, uint32_t c2,
@@ -341,7 +370,9 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
         valid[j] = pl < n_paths;
         const uint64_t gp = R.path_offset + pl;
         plo[j] = (uint32_t)gp;
-        phi[j] = (uint32_t)(gp >> 32);
+        // path_offset is a multiple of 64, so a tile never straddles 2^32: the high word is
+        // wave-uniform and the first Philox rounds run partly on the scalar unit
+        phi[j] = uniform((uint32_t)((R.path_offset + (uint64_t)(tile0 + (uint32_t)j) * kTile) >> 32));
         V[j] = R.v0_f;
         Y[j] = 0.0f;
         X[j] = 1.0f;
@@ -473,7 +504,7 @@ __device__ __forceinline__ void path_fp64(const PathArgs& a, uint32_t ri, uint32
     const uint64_t pl = (uint64_t)tile * kTile + (uint64_t)lane;
     const bool valid = pl < R.n_paths;
     const uint64_t gp = R.path_offset + pl;
-    const uint32_t plo = (uint32_t)gp, phi = (uint32_t)(gp >> 32);
+    const uint32_t plo = (uint32_t)gp, phi = uniform((uint32_t)((R.path_offset + (uint64_t)tile * kTile) >> 32));
     double V = R.v0, logX = R.log_S0, X = R.S0, Xp = R.S0, acc = 0.0;
     uint32_t wb0 = 0, wb1 = 0, k = 0;
     uint32_t cq = 0, ce = 0, cc = 0, ct = 0;

@@ -69,7 +69,7 @@ typedef struct hexo_request {
     double r;                            /* rate: used only with HEXO_DRIFT / HEXO_DISCOUNT */
     uint64_t n_paths;
     uint64_t seed;
-    uint64_t path_offset; /* first global path index (sharding, resumable chunks) */
+    uint64_t path_offset; /* first global path index (sharding, resumable chunks); a multiple of 64 */
     uint32_t steps;
     uint32_t n_expiries;
     const double* expiries;
@@ -85,8 +85,8 @@ typedef struct hexo_request {
  * reduced on its own and tiles are combined in a fixed order. */
 typedef struct hexo_tuning {
     uint32_t paths_per_thread; /* 1, 2 or 4 */
-    uint32_t block_threads;    /* 64, 128, 256, 512 or 1024 */
-    uint32_t waves_per_simd;   /* persistent-grid occupancy target, 1..8 (0 = from the occupancy API) */
+    uint32_t block_threads;    /* 64, 128 or 256 */
+    uint32_t waves_per_simd;   /* persistent-grid occupancy cap, 1..8 (0 = the occupancy API's maximum) */
     uint32_t reserved;
 } hexo_tuning;
 

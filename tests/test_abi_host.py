@@ -56,6 +56,7 @@ MUTATIONS = {
     "payoff_bits": lambda r: setattr(r, "payoffs", 8),
     "flags": lambda r: setattr(r, "flags", 64),
     "precision": lambda r: setattr(r, "precision", 7),
+    "path_offset_unaligned": lambda r: setattr(r, "path_offset", 100),
 }
 
 

@@ -228,3 +228,18 @@ def test_graph_replays_match_eager(ctx):
         p2, _ = b.results()
         np.testing.assert_array_equal(p, p2)
     b.close()
+
+
+def test_paths_window_and_offset(ctx, oracle):
+    """hexo_paths returns any window of a request's paths (unaligned starts included), and a
+    tile-aligned path_offset selects the same global paths."""
+    say("gpu paths window / path_offset")
+    req = small_requests()["c1_like"]
+    req.n_paths = 700
+    o = oracle_for(oracle, req)["finals"]
+    g = ctx.paths(req, 37, 101)
+    np.testing.assert_allclose(g, o[37:138], rtol=1e-12)
+    shifted = small_requests()["c1_like"]
+    shifted.n_paths, shifted.path_offset = 300, 128
+    g2 = ctx.paths(shifted, 0, 300)
+    np.testing.assert_allclose(g2, o[128:428], rtol=1e-12)
