@@ -32,8 +32,8 @@ PEAK_FP64_TFLOPS = 78.6
 F_STEP = 19      # moments m, s2, m^2, 1.5 m^2 (6); log-price step incl. exp2 (12); Asian sum (1)
 F_QUAD = 14      # quadratic variance branch without its normal
 F_EXP = 12       # exponential variance branch incl. its uniform
-F_CENTRAL = 35   # one AS241 normal, central region
-F_TAIL = 38      # one AS241 normal, tail region
+F_CENTRAL = 19   # one AS241 PPND7 normal, central region (q, r, 3+3 FMA rational, rcp, 2 mul)
+F_TAIL = 20      # one AS241 PPND7 normal, tail region (argument, log, sqrt, 3+2 FMA rational, rcp)
 F_SEG = 8        # per path and run of steps: Asian run sum folding and the new-dt accumulation
 F_FIN = 6        # per path and expiry: X_T interpolation and the Asian average
 F_PAYOFF = 1     # per path and output (FP32 subtraction; the sums are FP64)

@@ -80,12 +80,6 @@ __device__ __forceinline__ T horner8(const T (&c)[8], T r) {
     for (int i = 6; i >= 0; --i) acc = __builtin_fma(acc, r, c[i]);
     return acc;
 }
-__device__ __forceinline__ float horner8f(const float (&c)[8], float r) {
-    float acc = c[7];
-#pragma unroll
-    for (int i = 6; i >= 0; --i) acc = __builtin_fmaf(acc, r, c[i]);
-    return acc;
-}
 
 __device__ __forceinline__ double uniform_f64(uint32_t k) { return ((double)k + 0.5) * 0x1p-32; }
 // (k + 1/2) 2^-32 and 1 - that, in FP32, from the integer.
@@ -115,42 +109,37 @@ __device__ __forceinline__ double normal_f64(uint32_t k) {
     return (q < 0.0) ? -z : z;
 }
 
-// AS241 in FP32, driven by the integer. q = (k - 2^31 + 1/2) 2^-32 and the tail argument
-// min(u, 1-u) = (min(k, ~k) + 1/2) 2^-32 keep full relative precision in both tails. Both
-// rational approximations are evaluated and the region is selected: with 15% tail draws a
-// 64-lane wave almost always needs both anyway, and the select avoids the branch and the
-// exec-mask bookkeeping. The far-tail rational (r > 5) is unreachable from 32-bit inputs.
-__device__ __forceinline__ float normal_f32(uint32_t k) {
-    const int32_t sk = (int32_t)(k ^ 0x80000000u);
-    const float q = __builtin_fmaf((float)sk, 0x1p-32f, 0x1p-33f);
-    // central region |q| <= 0.425
-    const float rc = __builtin_fmaf(-q, q, 0.180625f);
-    const float nc = q * horner8f(As241<float>::a, rc);
-    const float dc = horner8f(As241<float>::b, rc);
-    // tail: r = sqrt(-ln(min(u,1-u))) - 1.6
-    const uint32_t mn = k ^ ~(uint32_t)(sk >> 31);
-    const float rr = __builtin_fmaf((float)mn, 0x1p-32f, 0x1p-33f);
-    const float rt = __builtin_amdgcn_sqrtf(-0.693147180559945309f * __builtin_amdgcn_logf(rr)) - 1.6f;
-    float nt = horner8f(As241<float>::c, rt);
-    const float dt = horner8f(As241<float>::d, rt);
-    nt = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, nt) ^ ((uint32_t)sk & 0x80000000u));
-    const bool central = __builtin_fabsf(q) <= 0.425f;
-    const float num = central ? nc : nt;
-    const float den = central ? dc : dt;
-    return num * __builtin_amdgcn_rcpf(den);
+// ---- AS241 PPND7 coefficients: the single-precision routine of the same algorithm (Wichura,
+// Appl. Statist. 37(3), 1988), accurate to about 1 part in 10^7 -- the resolution of FP32.
+// Central region (|q| <= 0.425): q A(r)/B(r), r = 0.180625 - q^2; tail (r = sqrt(-ln min(u,1-u))
+// <= 5): C(r - 1.6)/D(r - 1.6). The far-tail pair (r > 5) is unreachable from 32-bit words:
+// min(u, 1-u) >= 2^-33 gives r <= 4.78.
+struct Ppnd7 {
+    static constexpr float a[4] = {3.3871327179e0f, 5.0434271938e1f, 1.5929113202e2f, 5.9109374720e1f};
+    static constexpr float b[4] = {1.0f, 1.7895169469e1f, 7.8757757664e1f, 6.7187563600e1f};
+    static constexpr float c[4] = {1.4234372777e0f, 2.7568153900e0f, 1.3067284816e0f, 1.7023821103e-1f};
+    static constexpr float d[3] = {1.0f, 7.3700164250e-1f, 1.2021132975e-1f};
+};
+
+template <int N>
+__device__ __forceinline__ float hornerf(const float (&c)[N], float r) {
+    float acc = c[N - 1];
+#pragma unroll
+    for (int i = N - 2; i >= 0; --i) acc = __builtin_fmaf(acc, r, c[i]);
+    return acc;
 }
 
-// The two halves of normal_f32, for kernels that evaluate the tail region only where needed.
+// N(0,1) in FP32 from a 32-bit word through AS241 PPND7 (the FP32 kernel's inverse normal).
+// q = (k - 2^31 + 1/2) 2^-32 and the tail argument min(u, 1-u) = (min(k, ~k) + 1/2) 2^-32 are
+// formed from the integer, so both tails keep full relative precision (no 1 - u cancellation).
 // Central region: returns q A(r)/B(r) and sets `central` = |q| <= 0.425 (the value is meaningless
 // when `central` is false).
 __device__ __forceinline__ float normal_central_f32(uint32_t k, bool& central) {
     const int32_t sk = (int32_t)(k ^ 0x80000000u);
     const float q = __builtin_fmaf((float)sk, 0x1p-32f, 0x1p-33f);
     const float rc = __builtin_fmaf(-q, q, 0.180625f);
-    const float nc = q * horner8f(As241<float>::a, rc);
-    const float dc = horner8f(As241<float>::b, rc);
     central = __builtin_fabsf(q) <= 0.425f;
-    return nc * __builtin_amdgcn_rcpf(dc);
+    return (q * hornerf(Ppnd7::a, rc)) * __builtin_amdgcn_rcpf(hornerf(Ppnd7::b, rc));
 }
 // Tail region (|q| > 0.425): r = sqrt(-ln(min(u, 1-u))) - 1.6, C(r)/D(r) with the sign of q.
 __device__ __forceinline__ float normal_tail_f32(uint32_t k) {
@@ -158,14 +147,13 @@ __device__ __forceinline__ float normal_tail_f32(uint32_t k) {
     const uint32_t mn = k ^ ~(uint32_t)(sk >> 31);
     const float rr = __builtin_fmaf((float)mn, 0x1p-32f, 0x1p-33f);
     const float rt = __builtin_amdgcn_sqrtf(-0.693147180559945309f * __builtin_amdgcn_logf(rr)) - 1.6f;
-    const float z = horner8f(As241<float>::c, rt) * __builtin_amdgcn_rcpf(horner8f(As241<float>::d, rt));
+    const float z = hornerf(Ppnd7::c, rt) * __builtin_amdgcn_rcpf(hornerf(Ppnd7::d, rt));
     return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, z) ^ ((uint32_t)sk & 0x80000000u));
 }
-
-__device__ __forceinline__ bool normal_is_central(uint32_t k) {
-    const int32_t sk = (int32_t)(k ^ 0x80000000u);
-    const float q = __builtin_fmaf((float)sk, 0x1p-32f, 0x1p-33f);
-    return __builtin_fabsf(q) <= 0.425f;
+__device__ __forceinline__ float normal_f32(uint32_t k) {
+    bool central;
+    const float zc = normal_central_f32(k, central);
+    return central ? zc : normal_tail_f32(k);
 }
 
 }  // namespace hexo

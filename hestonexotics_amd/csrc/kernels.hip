@@ -247,7 +247,7 @@ __device__ __forceinline__ void tail_normals(const uint32_t (&w)[N], const bool 
     uint32_t total = 0;  // wave-uniform
 #pragma unroll
     for (int s = 0; s < N; ++s) {
-        const uint64_t mask = __ballot(t[s] ? 1 : 0);
+        const uint64_t mask = __builtin_amdgcn_ballot_w64(t[s]);
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
         slot[s] = total + rank;
         if (t[s]) q_in[slot[s]] = w[s];

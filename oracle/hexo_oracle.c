@@ -190,29 +190,77 @@ double orc_normal_f64_from_word(uint32_t k) {
     return orc_ppnd16_f64(orc_uniform_from_word(k), &f);
 }
 
-/* FP32 evaluation driven by the integer: q = u - 1/2 = (k - 2^31 + 1/2) 2^-32 and the tail
- * argument min(u, 1-u) = (min(k, ~k) + 1/2) 2^-32 are formed from k, so both tails keep full
- * relative precision. Same region logic and coefficients as AS241, float arithmetic. */
-float orc_normal_f32_from_word(uint32_t k) {
-    float fa[8], fb[8], fc[8], fd[8];
-    for (int i = 0; i < 8; ++i) {
-        fa[i] = (float)AS_A[i];
-        fb[i] = (float)AS_B[i];
-        fc[i] = (float)AS_C[i];
-        fd[i] = (float)AS_D[i];
+/* AS241's single-precision routine PPND7 (Wichura, Appl. Statist. 37(3), 1988; the reference
+ * ships only PPND16, src/as241.f90:15-119). Same regions as PPND16 (|q| <= 0.425, r <= 5) with
+ * lower-degree rationals, accurate to about 1 part in 10^7. The product's FP32 kernel uses it. */
+static const double P7_A[4] = {3.3871327179e0, 5.0434271938e1, 1.5929113202e2, 5.9109374720e1};
+static const double P7_B[4] = {1.0, 1.7895169469e1, 7.8757757664e1, 6.7187563600e1};
+static const double P7_C[4] = {1.4234372777e0, 2.7568153900e0, 1.3067284816e0, 1.7023821103e-1};
+static const double P7_D[3] = {1.0, 7.3700164250e-1, 1.2021132975e-1};
+static const double P7_E[4] = {6.6579051150e0, 3.0812263860e0, 4.2868294337e-1, 1.7337203997e-2};
+static const double P7_F[3] = {1.0, 2.4197894225e-1, 1.2258202635e-2};
+
+void orc_ppnd7_coefficients(double out[19]) {
+    int n = 0;
+    for (int i = 0; i < 4; ++i) out[n++] = P7_A[i];
+    for (int i = 1; i < 4; ++i) out[n++] = P7_B[i];
+    for (int i = 0; i < 4; ++i) out[n++] = P7_C[i];
+    for (int i = 1; i < 3; ++i) out[n++] = P7_D[i];
+    for (int i = 0; i < 4; ++i) out[n++] = P7_E[i];
+    for (int i = 1; i < 3; ++i) out[n++] = P7_F[i];
+}
+
+static inline double horner(const double* c, int n, double r) {
+    double acc = c[n - 1];
+    for (int i = n - 2; i >= 0; --i) acc = acc * r + c[i];
+    return acc;
+}
+static inline float hornerf(const double* c, int n, float r) {
+    float acc = (float)c[n - 1];
+    for (int i = n - 2; i >= 0; --i) acc = acc * r + (float)c[i];
+    return acc;
+}
+
+/* PPND7 evaluated in double, PPND16's region logic (src/as241.f90:85-118). */
+double orc_ppnd7_f64(double p, int* ifault) {
+    *ifault = 0;
+    double q = p - 0.5;
+    if (fabs(q) <= 0.425) {
+        double r = 0.180625 - q * q;
+        return q * horner(P7_A, 4, r) / horner(P7_B, 4, r);
     }
+    double r = (q < 0.0) ? p : 1.0 - p;
+    if (r <= 0.0) {
+        *ifault = 1;
+        return 0.0;
+    }
+    r = sqrt(-log(r));
+    double z;
+    if (r <= 5.0) {
+        r -= 1.6;
+        z = horner(P7_C, 4, r) / horner(P7_D, 3, r);
+    } else {
+        r -= 5.0;
+        z = horner(P7_E, 4, r) / horner(P7_F, 3, r);
+    }
+    return (q < 0.0) ? -z : z;
+}
+
+/* The FP32 kernel's normal: PPND7 in float, driven by the integer. q = u - 1/2 =
+ * (k - 2^31 + 1/2) 2^-32 and the tail argument min(u, 1-u) = (min(k, ~k) + 1/2) 2^-32 are formed
+ * from k, so both tails keep full relative precision. 32-bit words never reach the far tail:
+ * min(u, 1-u) >= 2^-33 gives r <= 4.78 < 5. */
+float orc_normal_f32_from_word(uint32_t k) {
     int32_t sk = (int32_t)(k ^ 0x80000000u);
     float q = ((float)sk + 0.5f) * 0x1p-32f;
     if (fabsf(q) <= 0.425f) {
         float r = 0.180625f - q * q;
-        return q * horner8f(fa, r) / horner8f(fb, r);
+        return q * hornerf(P7_A, 4, r) / hornerf(P7_B, 4, r);
     }
     uint32_t m = (sk < 0) ? k : ~k;
     float rr = ((float)m + 0.5f) * 0x1p-32f;
-    float r = sqrtf(-logf(rr));
-    /* 32-bit inputs never reach the far tail: min(u,1-u) >= 2^-33 gives r <= 4.78 < 5. */
-    r -= 1.6f;
-    float z = horner8f(fc, r) / horner8f(fd, r);
+    float r = sqrtf(-logf(rr)) - 1.6f;
+    float z = hornerf(P7_C, 4, r) / hornerf(P7_D, 3, r);
     return (sk < 0) ? -z : z;
 }
 
