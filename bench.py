@@ -29,9 +29,9 @@ PEAK_FP64_TFLOPS = 78.6
 
 # Algorithmic FP32 flops of the path kernel (kernels.hip), convention: add/sub/mul = 1, fma = 2,
 # exp2/log2/sqrt/rcp = 1; compares, selects, conversions and integer (RNG) work are not counted.
-F_STEP = 19      # moments m, s2, m^2, 1.5 m^2 (6); log-price step incl. exp2 (12); Asian sum (1)
-F_QUAD = 14      # quadratic variance branch without its normal
-F_EXP = 12       # exponential variance branch incl. its uniform
+F_STEP = 22      # moments m, s2, m^2, 1.5 m^2 (7); log-price step incl. exp2 (14); Asian sum (1)
+F_QUAD = 20      # quadratic variance branch without its normal (incl. a Newton-refined reciprocal)
+F_EXP = 20       # exponential variance branch incl. its uniform (two Newton-refined reciprocals)
 F_CENTRAL = 19   # one AS241 PPND7 normal, central region (q, r, 3+3 FMA rational, rcp, 2 mul)
 F_TAIL = 20      # one AS241 PPND7 normal, tail region (argument, log, sqrt, 3+2 FMA rational, rcp)
 F_SEG = 8        # per path and run of steps: Asian run sum folding and the new-dt accumulation

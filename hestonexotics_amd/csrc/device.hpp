@@ -81,6 +81,13 @@ __device__ __forceinline__ T horner8(const T (&c)[8], T r) {
     return acc;
 }
 
+// 1/x from the hardware estimate and one Newton step (error ~ulp^2 before the final rounding): the
+// estimate alone is not correctly rounded, and a systematic error in it biases scheme means.
+__device__ __forceinline__ float rcp_refined(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+
 __device__ __forceinline__ double uniform_f64(uint32_t k) { return ((double)k + 0.5) * 0x1p-32; }
 // (k + 1/2) 2^-32 and 1 - that, in FP32, from the integer.
 __device__ __forceinline__ float uniform_f32(uint32_t k) { return __builtin_fmaf((float)k, 0x1p-32f, 0x1p-33f); }

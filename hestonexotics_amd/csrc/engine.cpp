@@ -403,15 +403,15 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
             DevSeg& g = hs[seg_i++];
             const double dl = S.delta, k = q.kappa, th = q.theta, e = q.sigma, rho = q.rho;
             const double dd = std::exp(-k * dl), omd = -std::expm1(-k * dl);
-            g.d = (float)dd;
-            g.mth = (float)(th * omd);
+            g.omd = (float)omd;
+            g.th = (float)th;
             g.c1 = (float)(e * e * dd * omd / k);
             g.c2 = (float)(th * e * e * omd * omd / (2.0 * k));
             const double K0 = -rho * k * th / e * dl + d.r_drift * dl;
             const double kc = 0.5 * dl * (k * rho / e - 0.5);
             const double kd = rho / e;
             const double K3 = 0.5 * dl * (1.0 - rho * rho);
-            g.k0 = (float)(K0 * L2E);
+            g.g0 = (float)((K0 + 2.0 * (double)g.th * kc) * L2E);
             g.kc = (float)(kc * L2E);
             g.kd = (float)(kd * L2E);
             g.k3 = (float)(K3 * L2E * L2E);

@@ -223,7 +223,7 @@ __device__ __forceinline__ void count_flush(unsigned long long* counts, uint32_t
 // FP32 path kernel
 // ------------------------------------------------------------------------------------------
 struct SegF {
-    float d, mth, c1, c2, k0, kc, kd, k3;
+    float omd, th, c1, c2, g0, kc, kd, k3;
 };
 
 // Per-wave LDS scratch of the FP32 kernel: the tail-normal queue during the steps (2*PPT words in,
@@ -286,7 +286,7 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
     float z[2 * PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        m[j] = __builtin_fmaf(V[j], c.d, c.mth);
+        m[j] = __builtin_fmaf(c.omd, c.th - V[j], V[j]);
         s2[j] = __builtin_fmaf(V[j], c.c1, c.c2);
         m2[j] = m[j] * m[j];
         quad[j] = s2[j] < 1.5f * m2[j];
@@ -316,7 +316,10 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
             const float tm1 = tt - 1.0f;
             const float b2 = tm1 + __builtin_amdgcn_sqrtf(tt * tm1);
             const float b = __builtin_amdgcn_sqrtf(b2);
-            const float aq = m[j] * __builtin_amdgcn_rcpf(1.0f + b2);
+            // a = m / (1 + b^2) from the rounded b and a refined reciprocal, so that the scheme's
+            // mean E[V'] = a (b^2 + 1) = m holds up to unbiased rounding: a biased mean moves the
+            // variance's mean-reversion level by (relative bias) / (kappa dt)
+            const float aq = m[j] * rcp_refined(__builtin_fmaf(b, b, 1.0f));
             const float bz = b + z[2 * j];
             Vn = aq * bz * bz;
         } else {
@@ -325,13 +328,13 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
             const float sm = s2[j] + m2[j];
             const float lhs = one_minus_uniform_f32(wv[j]) * sm;  // (1-U) sm
             const float tm2 = m2[j] + m2[j];
-            const float lg = __builtin_amdgcn_logf(tm2 * __builtin_amdgcn_rcpf(lhs));
-            const float v = (sm * __builtin_amdgcn_rcpf(m[j])) * (0.5f * 0.693147180559945309f) * lg;
+            const float lg = __builtin_amdgcn_logf(tm2 * rcp_refined(lhs));
+            const float v = (sm * rcp_refined(m[j])) * (0.5f * 0.693147180559945309f) * lg;
             Vn = lhs < tm2 ? v : 0.0f;
         }
         const float sum = V[j] + Vn;
         const float dif = Vn - V[j];
-        const float inc = __builtin_fmaf(c.kd, dif, __builtin_fmaf(c.kc, sum, c.k0));
+        const float inc = __builtin_fmaf(c.kd, dif, __builtin_fmaf(c.kc, sum - (c.th + c.th), c.g0));
         const float sq = __builtin_amdgcn_sqrtf(c.k3 * sum);
         Y[j] = __builtin_fmaf(sq, z[2 * j + 1], Y[j] + inc);
         Xp[j] = X[j];
@@ -386,7 +389,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
     const uint32_t n_segs = R.n_segs;
     for (uint32_t s = 0; s < n_segs; ++s) {
         const DevSeg& S = a.segs[R.seg_begin + s];
-        const SegF c{S.d, S.mth, S.c1, S.c2, S.k0, S.kc, S.kd, S.k3};
+        const SegF c{S.omd, S.th, S.c1, S.c2, S.g0, S.kc, S.kd, S.k3};
         const uint32_t n = S.n_steps;
         uint32_t i = 0;
         if ((k & 1u) && n > 0) {  // second half of a block opened by the previous run

@@ -20,14 +20,18 @@ constexpr int kMaxPPT = 4;        // paths per thread (FP32 kernel)
 
 // One run of steps with a constant step width (the reference's dt = T_c / steps between two
 // expiry finalisations). FP32 constants are computed on the host in FP64 (per request, per run):
-//   m  = V d + mth                   (conditional mean, src/HSimulation.tpp:58-59)
+//   m  = V + omd (th - V)            (conditional mean, src/HSimulation.tpp:58-59; omd = 1 - e^-kappa dt)
 //   s2 = V c1 + c2                   (conditional variance, src/HSimulation.tpp:60)
-//   y += k0 + kc (V + V') + kd (V' - V) + sqrt(k3 (V + V')) Z      (log2 of X/S0)
-// with kc = (K1 + K2)/2, kd = (K2 - K1)/2 = rho/eps, k3 = K3 = K4 (src/HSimulation.tpp:75-80),
-// all scaled by log2(e) (k3 by log2(e)^2).
+//   y += g0 + kc (V + V' - 2 th) + kd (V' - V) + sqrt(k3 (V + V')) Z      (log2 of X/S0)
+// with kc = (K1 + K2)/2, kd = (K2 - K1)/2 = rho/eps, k3 = K3 = K4 and g0 = K0 + 2 th kc
+// (src/HSimulation.tpp:75-80), all scaled by log2(e) (k3 by log2(e)^2); th is theta rounded to
+// FP32 and g0 uses that same value. This is the reference's step re-associated so that rounding
+// the constants to FP32 moves neither fixed point: in the form V e^-kappa dt + theta(1 - e^-kappa dt)
+// the rounded e^-kappa dt shifts the mean-reversion level by O(ulp / (kappa dt)), and K0 and
+// K1 V + K2 V' nearly cancel, so their separate roundings add a drift of O(ulp K0) per step.
 struct DevSeg {
-    float d, mth, c1, c2;
-    float k0, kc, kd, k3;
+    float omd, th, c1, c2;
+    float g0, kc, kd, k3;
     float delta_f;  // Asian weight of accumulations inside the run
     float dnext_f;  // weight of the accumulation after the run's last step (new dt), 0 if none
     uint32_t n_steps;

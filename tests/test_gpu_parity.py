@@ -156,6 +156,33 @@ def test_fp32_prices_vs_fp64_identical_draws(ctx):
         assert rel.max() < 1e-4
 
 
+FP32_BIAS_CASES = {
+    # fast mean reversion over many steps (kappa dt = 0.015, 823 steps): a mean-reversion level or
+    # drift that moved with the FP32 rounding of e^-kappa dt or K0 would bias the price by ~7e-5
+    "fast_reversion": dict(v0=0.0422, theta=0.1744, rho=-0.796, kappa=12.1, sigma=0.178, expiries=[3.2657],
+                           strikes=[[80.0, 113.7]], steps=823, n_paths=40_000),
+    # stiff C5-like: kappa dt = 2.4e-4, 45% exponential-branch steps
+    "stiff": dict(v0=0.04, theta=0.04, rho=-0.9, kappa=0.5, sigma=0.9, expiries=[1.0], strikes=[[90.0, 100.0]],
+                  steps=2048, n_paths=20_000),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FP32_BIAS_CASES))
+def test_fp32_step_has_no_systematic_bias(ctx, name):
+    """FP32 vs FP64 on identical draws for parameters that amplify a systematic FP32 error: the
+    price error stays at the level of per-path rounding noise, far below 1e-4."""
+    say("gpu fp32 bias", name)
+    kw = FP32_BIAS_CASES[name]
+    req = H.Request(**kw, S0=100.0, seed=77, r=0.02, flags=H.DRIFT | H.DISCOUNT, payoffs=H.ASIAN | H.EUROPEAN,
+                    precision=H.FP64)
+    p64, _ = ctx.price(req)
+    req.precision = H.FP32
+    p32, _ = ctx.price(req)
+    rel = np.abs(p32 - p64) / p64
+    say("  rel err", rel)
+    assert rel.max() < 1.5e-5
+
+
 def test_batch_equals_single(ctx):
     say("gpu batch == single")
     reqs = list(small_requests().values())
