@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-fp64-scale", type=float, default=0.05)
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--c5-paths", type=int, default=2 ** 27, help="paths of the stiff precision-study request")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--profile", action="store_true",
@@ -245,6 +247,28 @@ def main():
                              "fp32_kernel_ms": k3, "fp64_value": v3d, "fp64_path_scale": args.c3_fp64_scale,
                              "fp32_over_fp64": v3 / v3d, "fp32_vs_fp64_max_rel_err": float(rel3.max())}
         log("C3:", out["c3_batched"])
+
+    if not args.no_c5:
+        # precision study on the stiff Asian call: FP32 stepping with FP64 or compensated-FP32
+        # reduction against full FP64, same draws; errors stated relative and in standard errors
+        c5 = {}
+        for name, prec in (("fp64", H.FP64), ("fp32_fp64_reduce", H.FP32), ("fp32_compensated", H.FP32_COMPENSATED)):
+            b5 = ctx.batch([W.c5(n_paths=args.c5_paths, precision=prec)])
+            w5, k5 = time_batch(ctx, b5, 1, 0)
+            p5, s5 = b5.results()
+            st5 = b5.stats()
+            b5.close()
+            c5[name] = {"price": float(p5[0]), "std_err": float(s5[0]), "s": w5, "kernel_ms": k5,
+                        "path_steps_per_s": W.path_steps([W.c5(n_paths=args.c5_paths)]) / w5}
+            log(f"C5 {name}: price {p5[0]:.6f} se {s5[0]:.2e} kernel {k5:.1f} ms")
+        ref = c5["fp64"]
+        for name in ("fp32_fp64_reduce", "fp32_compensated"):
+            d = c5[name]["price"] - ref["price"]
+            c5[name]["rel_err_vs_fp64"] = abs(d) / ref["price"]
+            c5[name]["err_in_se"] = abs(d) / ref["std_err"]
+        c5["workload"] = (f"C5 stiff Asian call (kappa 0.5, sigma 0.9, rho -0.9), {args.c5_paths} paths x 2048 "
+                          "steps, one request")
+        out["c5_precision"] = c5
 
     if args.serve_seconds > 0:
         reqs4, arr = W.c4(duration_s=args.serve_seconds, precision=H.FP32)

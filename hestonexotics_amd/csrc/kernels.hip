@@ -627,8 +627,12 @@ __global__ __launch_bounds__(256) void k_reduce_groups(ReduceArgs a, uint64_t to
     }
 }
 
+// One wavefront per output: lane l sums groups l, l + 64, l + 128, ... in order and the 64 lane
+// sums are combined by a fixed xor butterfly (every lane ends with the same bits). The order
+// depends only on the request's group count, never on the launch.
 __global__ __launch_bounds__(256) void k_reduce_final(ReduceArgs a, uint64_t total) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / (uint64_t)kTile;  // wave-uniform
+    const uint32_t lane = threadIdx.x % (uint32_t)kTile;
     if (t >= total) return;
     const uint32_t ri = find_req(a.out_end, a.n_req, t);
     const uint32_t o = (uint32_t)(t - (ri ? a.out_end[ri - 1] : 0ull));
@@ -638,10 +642,17 @@ __global__ __launch_bounds__(256) void k_reduce_final(ReduceArgs a, uint64_t tot
     if (R.precision == kPrecComp) {
         const float4* p = reinterpret_cast<const float4*>(a.groups);
         CompF cs{0.f, 0.f}, css{0.f, 0.f};
-        for (uint32_t g = 0; g < R.n_groups; ++g) {
+        for (uint32_t g = lane; g < R.n_groups; g += kTile) {
             const float4 v = p[(size_t)R.group_base + (size_t)g * n_out + o];
             cs.merge(v.x, v.y);
             css.merge(v.z, v.w);
+        }
+#pragma unroll
+        for (int m = 1; m < kTile; m <<= 1) {
+            const float s2 = __shfl_xor(cs.s, m), c2 = __shfl_xor(cs.c, m);
+            const float q2 = __shfl_xor(css.s, m), d2 = __shfl_xor(css.c, m);
+            cs.merge(s2, c2);
+            css.merge(q2, d2);
         }
         s = (double)cs.s + (double)cs.c;
         ss = (double)css.s + (double)css.c;
@@ -649,12 +660,18 @@ __global__ __launch_bounds__(256) void k_reduce_final(ReduceArgs a, uint64_t tot
         const double2* p = reinterpret_cast<const double2*>(a.groups);
         s = 0.0;
         ss = 0.0;
-        for (uint32_t g = 0; g < R.n_groups; ++g) {
+        for (uint32_t g = lane; g < R.n_groups; g += kTile) {
             const double2 v = p[(size_t)R.group_base + (size_t)g * n_out + o];
             s += v.x;
             ss += v.y;
         }
+#pragma unroll
+        for (int m = 1; m < kTile; m <<= 1) {
+            s += __shfl_xor(s, m);
+            ss += __shfl_xor(ss, m);
+        }
     }
+    if (lane != 0) return;
     const uint32_t opt = o % R.n_opts;
     const double disc = a.opts[R.opt_begin + opt].disc;
     const double scale = (R.precision == kPrecFP64) ? 1.0 : R.S0;
@@ -783,7 +800,8 @@ hipError_t launch_reduce(const ReduceArgs& a, uint64_t n_group_threads, uint64_t
     if (n_group_threads) k_reduce_groups<<<grid_for(n_group_threads, 256), 256, 0, st>>>(a, n_group_threads);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (n_out_threads) k_reduce_final<<<grid_for(n_out_threads, 256), 256, 0, st>>>(a, n_out_threads);
+    // one wavefront per output
+    if (n_out_threads) k_reduce_final<<<grid_for(n_out_threads * kTile, 256), 256, 0, st>>>(a, n_out_threads);
     return hipGetLastError();
 }
 
