@@ -39,6 +39,19 @@ F_FIN = 6        # per path and expiry: X_T interpolation and the Asian average
 F_PAYOFF = 1     # per path and output (FP32 subtraction; the sums are FP64)
 
 
+def pmc_traffic(paths: int, tuning: dict):
+    """HBM bytes per launch of the path kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (scaled by the copy-kernel calibration), or None when this run's
+    workload or launch shape is not the one profiled."""
+    f = os.path.join(ROOT, "profiles", "k_path_fp32_c2_grid_traffic.json")
+    if not os.path.exists(f):
+        return None
+    d = json.load(open(f))
+    if d["paths_per_maturity"] != paths or d["paths_per_thread"] != tuning["paths_per_thread"]:
+        return None
+    return float(d["hbm_bytes_per_launch"])
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -123,8 +136,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="profiling driver: 8 eager FP32 passes of the workload, then a plain copy of "
+                    help="profiling driver: --profile-passes eager FP32 passes of the workload, then a plain copy of "
                          "--copy-mib MiB (the known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--profile-passes", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=512)
     ap.add_argument("--sweep", action="store_true",
                     help="time the FP32 workload under every launch shape the autotuner considers, then exit")
@@ -142,10 +156,13 @@ def main():
     log("device", info)
     if args.profile:
         bp = ctx.batch(W.c2(n_paths=args.paths, precision=H.FP32))
-        for i in range(8):
+        times = []
+        for i in range(args.profile_passes):
             bp.run_eager()
             bp.sync()
-            log(f"profile pass {i}: path kernel {bp.stats()['path_kernel_ms']:.3f} ms")
+            times.append(bp.stats()["path_kernel_ms"])
+            log(f"profile pass {i}: path kernel {times[-1]:.3f} ms")
+        log(f"profile: mean path kernel {sum(times) / len(times):.3f} ms over {len(times)} passes (HIP events)")
         bp.close()
         ms = ctx.copy_probe(args.copy_mib << 20, reps=3)
         log(f"copy probe: {args.copy_mib} MiB in {ms:.3f} ms ({(2 * (args.copy_mib << 20)) / ms / 1e6:.1f} GB/s)")
@@ -214,7 +231,8 @@ def main():
                    "simulated_path_steps_per_step": b32.stats()["path_steps"], "tuning": tuning,
                    "device": info},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pmc_traffic(args.paths, tuning),
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)",
                      "kernel": "k_path_fp32", "kernel_ms": kms32, "algorithmic_flops_per_launch": flops,
                      "branch_counts": {"quadratic_steps": int(counts[0]), "exponential_steps": int(counts[1]),
                                        "central_normals": int(counts[2]), "tail_normals": int(counts[3])}},

@@ -234,24 +234,24 @@ struct WaveLds {
 };
 
 // AS241 tail evaluations gathered across the wave. Each lane holds N normals whose central-region
-// value is already in z[s]; where t[s] is set (|q| > 0.425) the tail rational is needed instead.
-// The wave packs those words into an LDS queue (ballot + mbcnt give each one a slot), evaluates
+// value is already in z[s]; where t[s] is set (|q| > 0.425) the tail rational is needed instead,
+// and mask[s] is the wave's ballot of t[s]. The wave packs those words into an LDS queue (the
+// ballot and mbcnt give each one a slot), evaluates
 // the queue densely 64 entries at a time, and each lane reads its results back. With 15% tail
 // draws this replaces N full-wave tail evaluations by ceil(0.15 * 64 N / 64) dense ones.
 template <int N>
-__device__ __forceinline__ void tail_normals(const uint32_t (&w)[N], const bool (&t)[N], float (&z)[N], uint32_t* q,
-                                             int lane) {
+__device__ __forceinline__ void tail_normals(const uint32_t (&w)[N], const bool (&t)[N], const uint64_t (&mask)[N],
+                                             float (&z)[N], uint32_t* q, int lane) {
     uint32_t* q_in = q;
     float* q_out = reinterpret_cast<float*>(q + N * kTile);
     uint32_t slot[N];
     uint32_t total = 0;  // wave-uniform
 #pragma unroll
     for (int s = 0; s < N; ++s) {
-        const uint64_t mask = __builtin_amdgcn_ballot_w64(t[s]);
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        slot[s] = total + rank;
+        // slot = total + number of lower lanes with a tail draw (mbcnt adds its accumulator)
+        slot[s] = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[s] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask[s], total));
         if (t[s]) q_in[slot[s]] = w[s];
-        total += (uint32_t)__builtin_popcountll(mask);
+        total += (uint32_t)__builtin_popcountll(mask[s]);
     }
     total = uniform(total);
     wave_sync_lds();
@@ -283,6 +283,7 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
     bool quad[PPT];
     uint32_t w[2 * PPT];
     bool t[2 * PPT];
+    uint64_t mk[2 * PPT];
     float z[2 * PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
@@ -297,6 +298,9 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
         w[2 * j + 1] = wx[j];
         t[2 * j] = quad[j] && !cv;
         t[2 * j + 1] = !cx;
+        // ballots of the comparisons themselves (a ballot of the combined bool costs two VALU ops)
+        mk[2 * j] = __builtin_amdgcn_ballot_w64(quad[j]) & __builtin_amdgcn_ballot_w64(!cv);
+        mk[2 * j + 1] = __builtin_amdgcn_ballot_w64(!cx);
         if constexpr (COUNT) {
             if (valid[j]) {
                 cnt.quad += quad[j] ? 1u : 0u;
@@ -306,7 +310,7 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
             }
         }
     }
-    tail_normals<2 * PPT>(w, t, z, q, lane);
+    tail_normals<2 * PPT>(w, t, mk, z, q, lane);
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         float Vn;
