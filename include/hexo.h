@@ -157,8 +157,9 @@ int hexo_rng_fill(hexo_ctx* ctx, uint64_t seed, uint32_t stream, uint64_t path0,
                   uint64_t word0, uint64_t n_words, uint32_t* out);
 /* Philox4x32-10 on the GPU: out[4i..4i+3] = philox(ctr[4i..], key[2i..]). */
 int hexo_philox4x32_10(hexo_ctx* ctx, const uint32_t* ctr, const uint32_t* key, uint64_t n, uint32_t* out);
-/* Word -> N(0,1) through AS241 on the GPU, in HEXO_FP64 or HEXO_FP32 arithmetic (replaces
- * RNG::setup_g with ppnd16, src/RNG.cpp:34-43, src/as241.f90:15-119). */
+/* Word -> N(0,1) through AS241 on the GPU: PPND16 in HEXO_FP64 arithmetic, or the FP32 kernel's
+ * PPND7 (AS241's single-precision routine) for HEXO_FP32 (replaces RNG::setup_g with ppnd16,
+ * src/RNG.cpp:34-43, src/as241.f90:15-119). */
 int hexo_normals(hexo_ctx* ctx, const uint32_t* words, uint64_t n, uint32_t precision, double* out);
 
 /* ---- launch shape ---- */
