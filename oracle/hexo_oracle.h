@@ -95,6 +95,12 @@ int orc_max_threads(void);
  * reference's release build). Strikes share expiry T. Returns 0 or a negative error. */
 int orc_swift_price(const double params[5], double S, double r, double T, uint32_t n_strikes, const double* strikes,
                     const double* swift_params, double trunc, double* prices, double* used_params);
+/* The same with the price gradient (SWIFT::price_opts_grad, src/SWIFT.cpp:92-97): grads[5 o + k]
+ * is d price_o / d params[k] with the SWIFT parameters held fixed, from the characteristic
+ * function evaluated in forward-mode dual arithmetic. grads may be NULL. */
+int orc_swift_price_grad(const double params[5], double S, double r, double T, uint32_t n_strikes,
+                         const double* strikes, const double* swift_params, double trunc, double* prices,
+                         double* grads, double* used_params);
 /* Heston characteristic function of the reference (src/HDistribution.cpp:12-38) at real u. */
 void orc_heston_chf(const double params[5], double tau, double u, double out[2]);
 

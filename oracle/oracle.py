@@ -84,6 +84,9 @@ def load(kind: str = "strict") -> C.CDLL:
         lib.orc_swift_price.argtypes = [_f64p, C.c_double, C.c_double, C.c_double, C.c_uint32, _f64p, C.c_void_p,
                                         C.c_double, _f64p, _f64p]
         lib.orc_swift_price.restype = C.c_int
+        lib.orc_swift_price_grad.argtypes = [_f64p, C.c_double, C.c_double, C.c_double, C.c_uint32, _f64p, C.c_void_p,
+                                             C.c_double, _f64p, C.c_void_p, _f64p]
+        lib.orc_swift_price_grad.restype = C.c_int
         lib.orc_heston_chf.argtypes = [_f64p, C.c_double, C.c_double, _f64p]
     lib._hexo_name = name
     _LIBS[kind] = lib
@@ -189,19 +192,24 @@ def price(params, S0: float, T, n_strikes, strikes, *, steps: int, n_paths: int,
     return out
 
 
-def swift_price(params, S: float, r: float, T: float, strikes, swift_params=None, trunc: float = 1e-7):
+def swift_price(params, S: float, r: float, T: float, strikes, swift_params=None, trunc: float = 1e-7,
+                grads: bool = False):
+    """SWIFT European call prices (and, with grads=True, their gradients, shape (n, 5) in the
+    order v0, theta, rho, kappa, sigma); returns (prices, used SWIFT parameters[, grads])."""
     lib = load()
     K = np.ascontiguousarray(np.atleast_1d(strikes), np.float64)
     prices = np.zeros(len(K), np.float64)
     used = np.zeros(8, np.float64)
+    g = np.zeros((len(K), 5), np.float64) if grads else None
     sp = None
     if swift_params is not None:
         sp_arr = np.ascontiguousarray(swift_params, np.float64)
         sp = sp_arr.ctypes.data
-    rc = lib.orc_swift_price(np.asarray(params, np.float64), S, r, T, len(K), K, sp, trunc, prices, used)
+    rc = lib.orc_swift_price_grad(np.asarray(params, np.float64), S, r, T, len(K), K, sp, trunc, prices,
+                                  g.ctypes.data if grads else None, used)
     if rc != 0:
-        raise ValueError(f"orc_swift_price failed: {rc}")
-    return prices, used
+        raise ValueError(f"orc_swift_price_grad failed: {rc}")
+    return (prices, used, g) if grads else (prices, used)
 
 
 def heston_chf(params, tau: float, u: float) -> complex:

@@ -31,6 +31,94 @@ static double complex chf(const double p[5], double tau, double complex u) {
     return cexp((kappa * vm * rho * tau * u * I) / sigma - a + (2.0 * dd * kappa * vm) / (sigma * sigma));
 }
 
+/* ---- price gradients: the same characteristic function in forward-mode dual arithmetic ----
+ * cd carries a value and its partial derivatives with respect to (v0, theta, rho, kappa, sigma);
+ * every operation applies the chain rule, so chf_dual returns chf and its exact gradient (up to
+ * rounding) without hand-derived formulas. The reference's analytic gradient
+ * (src/HDistribution.cpp:47-84) is checked against this through its golden values. */
+typedef struct {
+    double complex v;
+    double complex d[5];
+} cd;
+
+static cd cd_const(double complex v) {
+    cd r;
+    r.v = v;
+    for (int k = 0; k < 5; ++k) r.d[k] = 0;
+    return r;
+}
+static cd cd_var(double v, int k) {
+    cd r = cd_const(v);
+    r.d[k] = 1;
+    return r;
+}
+static cd cd_add(cd a, cd b) {
+    for (int k = 0; k < 5; ++k) a.d[k] += b.d[k];
+    a.v += b.v;
+    return a;
+}
+static cd cd_sub(cd a, cd b) {
+    for (int k = 0; k < 5; ++k) a.d[k] -= b.d[k];
+    a.v -= b.v;
+    return a;
+}
+static cd cd_mul(cd a, cd b) {
+    cd r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+static cd cd_scale(cd a, double complex c) {
+    a.v *= c;
+    for (int k = 0; k < 5; ++k) a.d[k] *= c;
+    return a;
+}
+static cd cd_div(cd a, cd b) {
+    cd r;
+    r.v = a.v / b.v;
+    for (int k = 0; k < 5; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+    return r;
+}
+/* unary f with derivative f'(a.v) */
+static cd cd_apply(cd a, double complex fv, double complex dfv) {
+    cd r;
+    r.v = fv;
+    for (int k = 0; k < 5; ++k) r.d[k] = dfv * a.d[k];
+    return r;
+}
+static cd cd_sqrt(cd a) {
+    const double complex s = csqrt(a.v);
+    return cd_apply(a, s, 0.5 / s);
+}
+static cd cd_exp(cd a) {
+    const double complex e = cexp(a.v);
+    return cd_apply(a, e, e);
+}
+static cd cd_log(cd a) { return cd_apply(a, clog(a.v), 1.0 / a.v); }
+static cd cd_sinh(cd a) { return cd_apply(a, csinh(a.v), ccosh(a.v)); }
+static cd cd_cosh(cd a) { return cd_apply(a, ccosh(a.v), csinh(a.v)); }
+
+/* chf(u) and its gradient; the expression is chf() above, step for step */
+static cd chf_dual(const double p[5], double tau, double u) {
+    const cd v0 = cd_var(p[0], 0), vm = cd_var(p[1], 1), rho = cd_var(p[2], 2), kappa = cd_var(p[3], 3),
+             sigma = cd_var(p[4], 4);
+    const cd iu = cd_const(I * u);
+    const cd xi = cd_add(kappa, cd_mul(cd_mul(sigma, rho), iu));
+    const cd fac = cd_const(u * u - I * u);
+    const cd d = cd_sqrt(cd_add(cd_mul(xi, xi), cd_mul(cd_mul(sigma, sigma), fac)));
+    const cd half = cd_scale(d, tau * 0.5);
+    const cd sh = cd_sinh(half), ch = cd_cosh(half);
+    const cd a1 = cd_mul(fac, sh);
+    const cd a2 = cd_add(cd_mul(cd_div(d, v0), ch), cd_mul(cd_div(xi, v0), sh));
+    const cd a = cd_div(a1, a2);
+    const cd num = cd_scale(d, 2.0);
+    const cd den = cd_add(cd_add(d, xi), cd_mul(cd_sub(d, xi), cd_exp(cd_scale(d, -tau))));
+    const cd dd = cd_add(cd_log(cd_div(num, den)), cd_scale(cd_sub(kappa, d), tau * 0.5));
+    const cd t1 = cd_div(cd_scale(cd_mul(cd_mul(kappa, vm), rho), tau * u * I), sigma);
+    const cd t3 = cd_div(cd_scale(cd_mul(cd_mul(dd, kappa), vm), 2.0), cd_mul(sigma, sigma));
+    return cd_exp(cd_add(cd_sub(t1, a), t3));
+}
+
 void orc_heston_chf(const double params[5], double tau, double u, double out[2]) {
     double complex z = chf(params, tau, u);
     out[0] = creal(z);
@@ -109,8 +197,9 @@ static void dynamic_params(const double p[5], double S, double r, double tau, do
 
 static inline int pmod(long a, long n) { return (int)(((a % n) + n) % n); }
 
-int orc_swift_price(const double params[5], double S, double r, double T, uint32_t n_strikes, const double* strikes,
-                    const double* swift_in, double trunc, double* prices, double* used) {
+int orc_swift_price_grad(const double params[5], double S, double r, double T, uint32_t n_strikes,
+                         const double* strikes, const double* swift_in, double trunc, double* prices, double* grads,
+                         double* used) {
     if (n_strikes == 0 || !(T > 0) || !(S > 0)) return -1;
     swift_params sp;
     if (swift_in) {
@@ -181,15 +270,32 @@ int orc_swift_price(const double params[5], double S, double r, double T, uint32
         coeff[j] = conj(acc) * sp.sqrt_exp2_m / (double)J;
     }
     /* price = Re sum_j chf(u_j) c_j e^{-rT} K e^{-i u_j x}, x = rT + log(S/K) (src/SWIFT.cpp:102-117) */
+    /* with grads: rows 1..5 are the partials of chf, as in the reference's 6 x J pricing
+     * matrix (src/SWIFT.cpp:106-110); the strike factors are the same for every row */
     const double disc = exp(-r * T);
-    double complex* pm = (double complex*)malloc(sizeof(double complex) * J);
-    for (unsigned j = 0; j < J; ++j) pm[j] = chf(params, T, u_of(&sp, j)) * coeff[j];
+    const int rows = grads ? 6 : 1;
+    double complex* pm = (double complex*)malloc(sizeof(double complex) * J * (size_t)rows);
+    if (!pm) return -3;
+    for (unsigned j = 0; j < J; ++j) {
+        if (grads) {
+            const cd c = chf_dual(params, T, u_of(&sp, j));
+            pm[j] = c.v * coeff[j];
+            for (int k = 0; k < 5; ++k) pm[(size_t)(k + 1) * J + j] = c.d[k] * coeff[j];
+        } else {
+            pm[j] = chf(params, T, u_of(&sp, j)) * coeff[j];
+        }
+    }
     for (uint32_t o = 0; o < n_strikes; ++o) {
         const double K = strikes[o];
         const double x = r * T + log(S / K);
-        double complex acc = 0;
-        for (unsigned j = 0; j < J; ++j) acc += pm[j] * (disc * K * cexp(-I * u_of(&sp, j) * x));
-        prices[o] = creal(acc);
+        double complex acc[6] = {0, 0, 0, 0, 0, 0};
+        for (unsigned j = 0; j < J; ++j) {
+            const double complex e = disc * K * cexp(-I * u_of(&sp, j) * x);
+            for (int row = 0; row < rows; ++row) acc[row] += pm[(size_t)row * J + j] * e;
+        }
+        prices[o] = creal(acc[0]);
+        if (grads)
+            for (int k = 0; k < 5; ++k) grads[(size_t)o * 5 + k] = creal(acc[k + 1]);
     }
     free(pm);
     free(pay_in);
@@ -198,4 +304,9 @@ int orc_swift_price(const double params[5], double S, double r, double T, uint32
     free(dens_in);
     free(coeff);
     return 0;
+}
+
+int orc_swift_price(const double params[5], double S, double r, double T, uint32_t n_strikes, const double* strikes,
+                    const double* swift_in, double trunc, double* prices, double* used) {
+    return orc_swift_price_grad(params, S, r, T, n_strikes, strikes, swift_in, trunc, prices, NULL, used);
 }

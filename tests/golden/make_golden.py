@@ -11,7 +11,8 @@ heston_closed_form.json  Heston European call prices from Lewis' single-integral
 geometric_asian_gbm.json  Black-Scholes closed forms used by the sigma -> 0 limit tests.
 
 The reference's own golden vectors (the 40 SWIFT prices of src/UnitTest.cpp:221-261 with their
-inputs, src/UnitTest.cpp:155-188) are data copied into swift_reference_prices.json by hand.
+inputs, src/UnitTest.cpp:155-188, and the 200 SWIFT price gradients of :278-477 with their index
+map :478) are data copied into swift_reference_prices.json.
 """
 import json
 import os
