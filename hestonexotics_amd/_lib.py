@@ -62,6 +62,22 @@ class hexo_server_stats(C.Structure):
                 ("path_steps_done", C.c_double)]
 
 
+class hexo_swift_grid(C.Structure):
+    _fields_ = [("m", C.c_uint32), ("exp2_m", C.c_uint32), ("sqrt_exp2_m", C.c_double), ("lower", C.c_double),
+                ("upper", C.c_double), ("k1", C.c_int32), ("k2", C.c_int32), ("J", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class hexo_swift_request(C.Structure):
+    _fields_ = [("v0", C.c_double), ("theta", C.c_double), ("rho", C.c_double), ("kappa", C.c_double),
+                ("sigma", C.c_double), ("S0", C.c_double), ("r", C.c_double), ("T", C.c_double),
+                ("n_strikes", C.c_uint32), ("flags", C.c_uint32), ("strikes", C.POINTER(C.c_double)),
+                ("trunc", C.c_double), ("grid", hexo_swift_grid)]
+
+
+SWIFT_FIXED_GRID = 1
+SWIFT_MAX_J = 16384
+
 _P = C.c_void_p
 _d = C.POINTER(C.c_double)
 _u32 = C.POINTER(C.c_uint32)
@@ -105,6 +121,9 @@ SIGNATURES = {
     "hexo_server_stop": (C.c_int, [_P]),
     "hexo_server_run_trace": (C.c_int, [_P, _req, _d, C.c_uint32, _d, _d, C.POINTER(hexo_ticket_info)]),
     "hexo_inject_fault": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "hexo_swift_grid_for": (C.c_int, [C.POINTER(hexo_swift_request), C.POINTER(hexo_swift_grid)]),
+    "hexo_swift_price": (C.c_int, [_P, C.POINTER(hexo_swift_request), C.c_uint32, _d, _d,
+                                   C.POINTER(hexo_swift_grid)]),
 }
 
 _lib: C.CDLL | None = None

@@ -16,12 +16,59 @@ from typing import Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import (ASIAN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64, TRAPEZOID, HexoError,
-                   hexo_batch_stats, hexo_request, hexo_server_config, hexo_server_stats, hexo_ticket_info,
-                   hexo_tuning)
+from ._lib import (ASIAN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64, SWIFT_FIXED_GRID, TRAPEZOID,
+                   HexoError, hexo_batch_stats, hexo_request, hexo_server_config, hexo_server_stats,
+                   hexo_swift_grid, hexo_swift_request, hexo_ticket_info, hexo_tuning)
 
 __all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "DRIFT", "DISCOUNT", "TRAPEZOID", "FP64", "FP32",
-           "FP32_COMPENSATED", "HexoError", "validate", "time_grid"]
+           "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "SwiftChain", "swift_grid"]
+
+GRID_FIELDS = ("m", "exp2_m", "sqrt_exp2_m", "lower", "upper", "k1", "k2", "J")
+
+
+@dataclass
+class SwiftChain:
+    """One expiry chain for the SWIFT European pricer (hexo_swift_request). grid: None (chosen from
+    the model and the strikes, src/SWIFT.cpp:21-35) or the 8 SwiftParameters values
+    (m, 2^m, sqrt(2^m), lower, upper, k1, k2, J) to use as given."""
+    v0: float
+    theta: float
+    rho: float
+    kappa: float
+    sigma: float
+    S0: float
+    T: float
+    strikes: Sequence[float]
+    r: float = 0.0
+    trunc: float = 1e-10
+    grid: Sequence[float] | None = None
+
+    def to_c(self):
+        q = hexo_swift_request()
+        for f in ("v0", "theta", "rho", "kappa", "sigma", "S0", "r", "T", "trunc"):
+            setattr(q, f, float(getattr(self, f)))
+        self._k = np.ascontiguousarray(self.strikes, np.float64)  # kept alive with the struct's owner
+        q.n_strikes = len(self._k)
+        q.strikes = self._k.ctypes.data_as(C.POINTER(C.c_double))
+        if self.grid is not None:
+            q.flags = SWIFT_FIXED_GRID
+            for f, v in zip(GRID_FIELDS, self.grid):
+                setattr(q.grid, f, type(getattr(q.grid, f))(v))
+        return q
+
+
+def _grid_tuple(g: hexo_swift_grid):
+    return tuple(getattr(g, f) for f in GRID_FIELDS)
+
+
+def swift_grid(chain: SwiftChain):
+    """The SWIFT grid the pricer chooses for a chain (host only)."""
+    g = hexo_swift_grid()
+    q = chain.to_c()
+    rc = _lib.load().hexo_swift_grid_for(C.byref(q), C.byref(g))
+    if rc != 0:
+        raise HexoError(rc, "hexo_swift_grid_for")
+    return _grid_tuple(g)
 
 
 @dataclass
@@ -266,6 +313,25 @@ class Context:
         out = np.zeros(w.size, np.float64)
         self._check(self.lib.hexo_normals(self.h, w.ctypes.data_as(C.POINTER(C.c_uint32)), w.size, precision,
                                           _dp(out)))
+        return out
+
+    # ---- SWIFT European pricer ----
+    def swift_price(self, chains: Sequence[SwiftChain], grads: bool = False):
+        """European call prices of each chain (and gradients, shape (n_strikes, 5) in the order
+        v0, theta, rho, kappa, sigma). Returns a list of (prices, grads or None, grid) per chain."""
+        arr = (hexo_swift_request * len(chains))()
+        for i, ch in enumerate(chains):
+            arr[i] = ch.to_c()
+        n = sum(len(ch.strikes) for ch in chains)
+        p = np.zeros(n, np.float64)
+        g = np.zeros(5 * n, np.float64) if grads else None
+        used = (hexo_swift_grid * len(chains))()
+        self._check(self.lib.hexo_swift_price(self.h, arr, len(chains), _dp(p), _dp(g) if grads else None, used))
+        out, o = [], 0
+        for i, ch in enumerate(chains):
+            k = len(ch.strikes)
+            out.append((p[o:o + k], g[5 * o:5 * (o + k)].reshape(k, 5) if grads else None, _grid_tuple(used[i])))
+            o += k
         return out
 
     # ---- tuning / measurement ----

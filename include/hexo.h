@@ -218,6 +218,48 @@ int hexo_server_run_trace(hexo_ctx* ctx, const hexo_request* reqs, const double*
  * HEXO_ERR_INJECTED; used by tests of failure handling. kind 0 = launch, 1 = admission. */
 int hexo_inject_fault(hexo_ctx* ctx, uint32_t kind, uint32_t count);
 
+/* ---- SWIFT European pricer (FP64) ----
+ * Replaces SWIFT / SwiftParameters / HDistribution::chf_chf_grad (src/SWIFT.cpp:18-118,
+ * src/inc/SWIFT.h:22-160, src/HDistribution.cpp:8-84): European call prices of one expiry chain
+ * from the Heston characteristic function on Shannon wavelets, optionally with the gradient in the
+ * five Heston parameters (SWIFT::price_opts_grad, the calibration Jacobian). Many chains run in one
+ * call. */
+typedef struct hexo_swift_grid {   /* SwiftParameters (src/inc/SWIFT.h:22-31) */
+    uint32_t m;                     /* wavelet scale */
+    uint32_t exp2_m;                /* 2^m */
+    double sqrt_exp2_m;             /* sqrt(2^m) */
+    double lower, upper;            /* truncation range of the log-price density */
+    int32_t k1, k2;                 /* wavelet index range */
+    uint32_t J;                     /* Fourier size, a power of two, <= HEXO_SWIFT_MAX_J */
+    uint32_t reserved;              /* must be 0 */
+} hexo_swift_grid;
+
+#define HEXO_SWIFT_MAX_J 16384u
+#define HEXO_SWIFT_FIXED_GRID 1u    /* use hexo_swift_request.grid as given (the reference's
+                                       predefined SwiftParameters); otherwise it is chosen from
+                                       the model and the strikes (src/SWIFT.cpp:21-35) */
+
+typedef struct hexo_swift_request {
+    double v0, theta, rho, kappa, sigma; /* HParams order (src/inc/HDistribution.h:9-14) */
+    double S0;                           /* spot */
+    double r;                            /* risk-free rate (HDistribution::risk_free) */
+    double T;                            /* expiry in years (HDistribution::tau) */
+    uint32_t n_strikes;
+    uint32_t flags;                      /* HEXO_SWIFT_FIXED_GRID or 0 */
+    const double* strikes;               /* n_strikes, > 0 */
+    double trunc;                        /* truncation precision of the dynamic grid (the reference
+                                            uses 1e-10, or 1e-7 when built with FASTMATH) */
+    hexo_swift_grid grid;                /* with HEXO_SWIFT_FIXED_GRID */
+} hexo_swift_request;
+
+/* The grid the pricer would choose for a request (host only; SwiftParameters ctor). */
+int hexo_swift_grid_for(const hexo_swift_request* req, hexo_swift_grid* out);
+/* Prices n requests (SWIFT::price_opts, src/SWIFT.cpp:86-91). prices: sum of n_strikes, chain
+ * order. grads: NULL, or 5 per price in the order v0, theta, rho, kappa, sigma
+ * (SWIFT::price_opts_grad, src/SWIFT.cpp:92-97). grids_used: NULL or n. Synchronous. */
+int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, uint32_t n, double* prices, double* grads,
+                     hexo_swift_grid* grids_used);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,3 +116,24 @@ def test_workload_generators_are_seeded():
     reqs, t = W.c4(duration_s=2.0, seed=1)
     assert np.all(np.diff(t) > 0) and 300 < len(reqs) < 500
     assert W.path_steps(W.c2()) == 824_000_000
+
+
+def test_swift_grid_matches_oracle_and_rejects_bad_chains():
+    """The host-side SWIFT grid (SwiftParameters, src/SWIFT.cpp:21-35) equals the oracle's."""
+    from oracle import oracle as O
+    for p, S, r, T, K in [([0.04, 0.04, -0.7, 2.0, 0.3], 100.0, 0.02, 1.0, list(np.linspace(70, 130, 64))),
+                          ([0.05, 0.04, -0.6, 1.5, 0.5], 100.0, 0.01, 0.75, [80.0, 100.0, 125.0])]:
+        g = H.swift_grid(H.SwiftChain(*p, S0=S, T=T, strikes=K, r=r, trunc=1e-10))
+        _, used = O.swift_price(p, S, r, T, K, trunc=1e-10)
+        assert [g[i] for i in (0, 1, 5, 6, 7)] == [used[i] for i in (0, 1, 5, 6, 7)]
+        assert np.allclose(g, used, rtol=1e-13, atol=0)
+    ok = dict(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0, T=1.0, strikes=[100.0])
+    for k, v in [("sigma", 0.0), ("rho", 1.0), ("T", 0.0), ("strikes", [0.0]), ("trunc", 0.0)]:
+        with pytest.raises(H.HexoError):
+            H.swift_grid(H.SwiftChain(**{**ok, k: v}))
+    for grid in ([5, 32, 5.65, -1.0, 1.0, 10, 5, 128],      # k1 > k2
+                 [5, 32, 5.65, -1.0, 1.0, -300, 300, 128],  # range beyond 4J
+                 [5, 32, 5.65, -1.0, 1.0, -10, 10, 100],    # J not a power of two
+                 [5, 16, 5.65, -1.0, 1.0, -10, 10, 128]):   # 2^m inconsistent
+        with pytest.raises(H.HexoError):
+            H.swift_grid(H.SwiftChain(**ok, grid=grid))
