@@ -115,6 +115,55 @@ def cpu_baseline(reqs, target_s: float):
                                                           f"{s1:.2f} s"}}
 
 
+def swift_chains(n_sets: int, seed: int = 3):
+    """A calibration-shaped SWIFT batch: n_sets Heston parameter sets, each priced on the grid
+    workload's 8 maturities x 64 strikes (the calls one Levenberg-Marquardt Jacobian needs)."""
+    import hestonexotics_amd as H
+    from hestonexotics_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    strikes = list(np.linspace(70.0, 130.0, 64))
+    out = []
+    for _ in range(n_sets):
+        v0, th = rng.uniform(0.02, 0.08, 2)
+        rho, kappa, sigma = rng.uniform(-0.9, -0.3), rng.uniform(0.5, 4.0), rng.uniform(0.2, 0.8)
+        out += [H.SwiftChain(v0, th, rho, kappa, sigma, S0=100.0, T=T, strikes=strikes, r=0.02)
+                for T in W.C2_MATURITIES]
+    return out
+
+
+def swift_leg(ctx, n_sets: int, reps: int = 5):
+    """GPU SWIFT prices + Jacobian for a calibration batch; the oracle (FP64, one thread) on a
+    sample of the same chains as the CPU baseline; GPU vs oracle error on that sample."""
+    from oracle import oracle as O
+    import hestonexotics_amd as H
+    chains = swift_chains(n_sets)
+    n_prices = sum(len(c.strikes) for c in chains)
+    call = H.SwiftCall(ctx, chains, grads=True)
+    call.run()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call.run()  # the C entry point: grids, packing, upload, kernels, results
+    wall = (time.perf_counter() - t0) / reps
+    res = call.split()
+    sample = chains[:8]
+    t1 = time.perf_counter()
+    ref = [O.swift_price([c.v0, c.theta, c.rho, c.kappa, c.sigma], c.S0, c.r, c.T, c.strikes, trunc=c.trunc,
+                         grads=True) for c in sample]
+    cpu_s = time.perf_counter() - t1
+    err_p = max(float(np.max(np.abs(r[0] - o[0]) / c.S0)) for r, o, c in zip(res, ref, sample))
+    err_g = max(float(np.max(np.abs(r[1] - o[2]) / c.S0)) for r, o, c in zip(res, ref, sample))
+    Js = [r[2][7] for r in res]
+    return {"workload": f"{n_sets} Heston parameter sets x 8 maturities x 64 strikes, prices + 5-parameter "
+                        "gradients, dynamic grids (trunc 1e-10)",
+            "chains": len(chains), "prices": n_prices, "ms_per_call": wall * 1e3,
+            "chains_per_s": len(chains) / wall, "prices_with_grads_per_s": n_prices / wall,
+            "J_min": int(min(Js)), "J_max": int(max(Js)),
+            "max_abs_err_vs_oracle_over_S": {"prices": err_p, "gradients": err_g},
+            "cpu_baseline": {"value": len(sample) / cpu_s, "unit": "chains/s (prices + gradients)", "cores": 1,
+                             "kind": "port", "sample": f"oracle SWIFT with dual-number gradients, {len(sample)} "
+                                                       f"chains in {cpu_s:.2f} s"}}
+
+
 def time_batch(ctx, batch, steps: int, warmup: int):
     for _ in range(warmup):
         batch.run_eager()
@@ -151,12 +200,16 @@ def main():
     ap.add_argument("--c3-fp64-scale", type=float, default=0.05)
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c5-paths", type=int, default=2 ** 27, help="paths of the stiff precision-study request")
+    ap.add_argument("--no-swift", action="store_true")
+    ap.add_argument("--swift-params", type=int, default=64, help="parameter sets of the SWIFT calibration batch")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="profiling driver: --profile-passes eager FP32 passes of the workload, then a plain copy of "
                          "--copy-mib MiB (the known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--profile-passes", type=int, default=20)
+    ap.add_argument("--profile-swift", action="store_true",
+                    help="profiling driver for the SWIFT pricer: --profile-passes calls on the calibration batch")
     ap.add_argument("--copy-mib", type=int, default=512)
     ap.add_argument("--sweep", action="store_true",
                     help="time the FP32 workload under every launch shape the autotuner considers, then exit")
@@ -184,6 +237,15 @@ def main():
         bp.close()
         ms = ctx.copy_probe(args.copy_mib << 20, reps=3)
         log(f"copy probe: {args.copy_mib} MiB in {ms:.3f} ms ({(2 * (args.copy_mib << 20)) / ms / 1e6:.1f} GB/s)")
+        ctx.close()
+        return
+    if args.profile_swift:
+        import hestonexotics_amd as H
+        call = H.SwiftCall(ctx, swift_chains(args.swift_params), grads=True)
+        for i in range(args.profile_passes):
+            t0 = time.perf_counter()
+            call.run()
+            log(f"profile swift pass {i}: {1e3 * (time.perf_counter() - t0):.3f} ms")
         ctx.close()
         return
     if args.sweep:
@@ -308,6 +370,10 @@ def main():
         c5["workload"] = (f"C5 stiff Asian call (kappa 0.5, sigma 0.9, rho -0.9), {args.c5_paths} paths x 2048 "
                           "steps, one request")
         out["c5_precision"] = c5
+
+    if not args.no_swift:
+        out["swift"] = swift_leg(ctx, args.swift_params)
+        log("SWIFT:", out["swift"])
 
     if args.serve_seconds > 0:
         reqs4, arr = W.c4(duration_s=args.serve_seconds, precision=H.FP32)

@@ -3,7 +3,7 @@
 The product is libhexo.so (hestonexotics_amd/csrc); this package is its Python binding.
 """
 from .api import (ASIAN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64, TRAPEZOID, Batch, Context,
-                  HexoError, Request, SwiftChain, swift_grid, time_grid, validate)
+                  HexoError, Request, SwiftCall, SwiftChain, swift_grid, time_grid, validate)
 
 __all__ = ["ASIAN", "DISCOUNT", "DRIFT", "EUROPEAN", "FP32", "FP32_COMPENSATED", "FP64", "TRAPEZOID", "Batch",
-           "Context", "HexoError", "Request", "SwiftChain", "swift_grid", "time_grid", "validate"]
+           "Context", "HexoError", "Request", "SwiftCall", "SwiftChain", "swift_grid", "time_grid", "validate"]

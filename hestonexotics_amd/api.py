@@ -21,7 +21,7 @@ from ._lib import (ASIAN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP6
                    hexo_swift_grid, hexo_swift_request, hexo_ticket_info, hexo_tuning)
 
 __all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "DRIFT", "DISCOUNT", "TRAPEZOID", "FP64", "FP32",
-           "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "SwiftChain", "swift_grid"]
+           "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "SwiftChain", "SwiftCall", "swift_grid"]
 
 GRID_FIELDS = ("m", "exp2_m", "sqrt_exp2_m", "lower", "upper", "k1", "k2", "J")
 
@@ -319,20 +319,8 @@ class Context:
     def swift_price(self, chains: Sequence[SwiftChain], grads: bool = False):
         """European call prices of each chain (and gradients, shape (n_strikes, 5) in the order
         v0, theta, rho, kappa, sigma). Returns a list of (prices, grads or None, grid) per chain."""
-        arr = (hexo_swift_request * len(chains))()
-        for i, ch in enumerate(chains):
-            arr[i] = ch.to_c()
-        n = sum(len(ch.strikes) for ch in chains)
-        p = np.zeros(n, np.float64)
-        g = np.zeros(5 * n, np.float64) if grads else None
-        used = (hexo_swift_grid * len(chains))()
-        self._check(self.lib.hexo_swift_price(self.h, arr, len(chains), _dp(p), _dp(g) if grads else None, used))
-        out, o = [], 0
-        for i, ch in enumerate(chains):
-            k = len(ch.strikes)
-            out.append((p[o:o + k], g[5 * o:5 * (o + k)].reshape(k, 5) if grads else None, _grid_tuple(used[i])))
-            o += k
-        return out
+        return SwiftCall(self, chains, grads).run().split()
+
 
     # ---- tuning / measurement ----
     def get_tuning(self) -> dict:
@@ -407,3 +395,34 @@ class Context:
         rc = self.lib.hexo_server_run_trace(self.h, arr, _dp(at), len(reqs), _dp(p), _dp(s), info)
         infos = [{k: getattr(info[i], k) for k, _ in info[i]._fields_} for i in range(len(reqs))]
         return rc, p[:n], s[:n], infos
+
+
+class SwiftCall:
+    """One hexo_swift_price call with its C request array and output buffers built once, so that
+    repeated calls (a calibration loop, a benchmark) go straight to the C ABI."""
+
+    def __init__(self, ctx: Context, chains: Sequence[SwiftChain], grads: bool = False):
+        self.ctx, self.chains, self.grads = ctx, list(chains), grads
+        self.arr = (hexo_swift_request * len(self.chains))()
+        for i, ch in enumerate(self.chains):
+            self.arr[i] = ch.to_c()
+        n = sum(len(ch.strikes) for ch in self.chains)
+        self.prices = np.zeros(n, np.float64)
+        self.grad_buf = np.zeros(5 * n, np.float64) if grads else None
+        self.used = (hexo_swift_grid * len(self.chains))()
+        self._pp = _dp(self.prices)
+        self._gp = _dp(self.grad_buf) if grads else None
+
+    def run(self) -> "SwiftCall":
+        c = self.ctx
+        c._check(c.lib.hexo_swift_price(c.h, self.arr, len(self.chains), self._pp, self._gp, self.used))
+        return self
+
+    def split(self):
+        out, o = [], 0
+        for i, ch in enumerate(self.chains):
+            k = len(ch.strikes)
+            g = self.grad_buf[5 * o:5 * (o + k)].reshape(k, 5).copy() if self.grads else None
+            out.append((self.prices[o:o + k].copy(), g, _grid_tuple(self.used[i])))
+            o += k
+        return out

@@ -136,6 +136,7 @@ int hexo_ctx_destroy(hexo_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
     }
+    if (ctx->swift_dev) (void)hipFree(ctx->swift_dev);
     delete ctx;
     return HEXO_OK;
 }

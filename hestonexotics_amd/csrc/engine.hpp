@@ -119,6 +119,11 @@ struct hexo_ctx {
     std::string last_error;
     struct hexo_server_state* server = nullptr;  // owned; see server.cpp
     std::unordered_set<struct hexo_batch*> batches;  // live batches; released by hexo_ctx_destroy
+    // SWIFT pricer workspace, grown on demand and kept for the context's lifetime (no per-call
+    // allocation, which would also synchronise the device)
+    void* swift_dev = nullptr;
+    size_t swift_dev_bytes = 0;
+    hexo::PinnedBuf swift_host;
     int set_error(int code, const std::string& msg) {
         last_error = msg;
         return code;

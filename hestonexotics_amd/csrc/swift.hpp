@@ -11,6 +11,12 @@ struct Cplx {
     double re, im;
 };
 
+// Chains with J up to this size run both transforms as radix-2 FFTs of length 2J in one
+// workgroup's LDS (16 B x 2J = 128 KiB); larger ones use the direct-sum kernels.
+constexpr uint32_t kSwiftFftMaxJ = 4096;
+// Strikes of one chain summed by one workgroup of the final contraction.
+constexpr int kSwiftStrikeBlock = 16;
+
 // One chain (expiry) of a SWIFT call: model, grid and the offsets of its slices in the call's
 // device region. Built on the host by swift_host.cpp.
 struct DevSwift {
@@ -20,9 +26,9 @@ struct DevSwift {
     int32_t k1, k2;
     uint32_t J, n_strikes;
     uint32_t rows;  // 1 (prices) or 6 (prices and the five partials)
-    uint32_t pad0;
+    uint32_t log2_2J;
     uint64_t strike_off;  // doubles
-    uint64_t tw_off;      // Cplx, 4J: e^{2 pi i k / 4J}
+    uint64_t tw_off;      // Cplx, 4J: e^{2 pi i k / 4J}; shared by the chains with the same J
     uint64_t pay_off;     // Cplx, J: payoff coefficients
     uint64_t dens_off;    // doubles, k2 - k1 + 1
     uint64_t coeff_off;   // Cplx, J: density coefficients
@@ -30,9 +36,19 @@ struct DevSwift {
     uint64_t out_off;     // doubles: prices; grads at 5x that offset in the grads array
 };
 
+// One twiddle table (per distinct J of the call).
+struct SwiftTable {
+    uint32_t J;
+    uint32_t pad0;
+    uint64_t off;  // Cplx
+};
+
 struct SwiftArgs {
     const DevSwift* reqs;
     uint32_t n_req;
+    uint32_t n_tables;
+    const SwiftTable* tables;
+    const uint32_t* fft_chains;  // chains whose transforms run as LDS FFTs
     const double* strikes;
     Cplx* tw;
     Cplx* pay;
@@ -41,16 +57,18 @@ struct SwiftArgs {
     Cplx* pm;
     double* prices;
     double* grads;  // null when no request asks for gradients
-    // inclusive prefix sums over requests of the per-kernel thread (or wave) counts
-    const uint64_t* tw_end;      // 4J
-    const uint64_t* dens_end;    // k2 - k1 + 1
-    const uint64_t* j_end;       // J
-    const uint64_t* strike_end;  // n_strikes (one wavefront each)
+    // inclusive prefix sums of the per-kernel thread (or wavefront) counts
+    const uint64_t* tab_end;     // over tables: 4J
+    const uint64_t* j_end;       // over chains: J
+    const uint64_t* dens_end;    // over chains: k2 - k1 + 1 for direct-sum chains, else 0
+    const uint64_t* dcoef_end;   // over chains: J for direct-sum chains, else 0
+    const uint64_t* sblk_end;    // over chains: groups of kSwiftStrikeBlock strikes (one workgroup each)
 };
 
-// Totals of the four index spaces (the last entries of the prefix sums).
 struct SwiftTotals {
-    uint64_t tw, dens, j, strikes;
+    uint64_t tw, j, dens, dcoef, sblocks;
+    uint32_t n_fft;         // chains in fft_chains
+    uint32_t fft_lds_bytes; // 16 x 2 x (largest J among them)
 };
 
 hipError_t launch_swift(const SwiftArgs& a, const SwiftTotals& t, hipStream_t st);

@@ -3,10 +3,12 @@
 // into one device region, and the C entry points hexo_swift_grid_for / hexo_swift_price.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hexo.h"
@@ -186,15 +188,30 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
     if (!reqs || !prices || n == 0) return ctx->set_error(HEXO_ERR_INVALID, "requests and prices must be non-NULL");
     std::string err;
     std::vector<hexo_swift_grid> grids(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        const int rc = grid_of(reqs[i], grids[i], err);
-        if (rc) return ctx->set_error(rc, "swift request " + std::to_string(i) + ": " + err);
+    {
+        // the dynamic grids (a few characteristic-function evaluations each) on host threads
+        std::vector<int> rcs(n, HEXO_OK);
+        std::vector<std::string> errs(n);
+        const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+        const uint32_t n_thr = std::min<uint32_t>(std::min<uint32_t>(hw, 16u), std::max<uint32_t>(1u, n / 32u));
+        auto work = [&](uint32_t t) {
+            for (uint32_t i = t; i < n; i += n_thr) rcs[i] = grid_of(reqs[i], grids[i], errs[i]);
+        };
+        std::vector<std::thread> pool;
+        for (uint32_t t = 1; t < n_thr; ++t) pool.emplace_back(work, t);
+        work(0);
+        for (auto& th : pool) th.join();
+        for (uint32_t i = 0; i < n; ++i)
+            if (rcs[i]) return ctx->set_error(rcs[i], "swift request " + std::to_string(i) + ": " + errs[i]);
     }
-    // host image of the device region: records, strikes, prefix sums; then the work arrays
+    // host image of the device region: records, tables, FFT chain list, prefix sums, strikes;
+    // then the work arrays
     std::vector<hexo::DevSwift> recs(n);
-    std::vector<uint64_t> ends(4 * (size_t)n);
-    uint64_t n_strk = 0, n_tw = 0, n_pay = 0, n_dens = 0, n_pm = 0;
-    hexo::SwiftTotals tot{0, 0, 0, 0};
+    std::vector<hexo::SwiftTable> tables;
+    std::vector<uint32_t> fft_chains;
+    std::vector<uint64_t> ends(4 * (size_t)n);  // j, dens, dcoef, strike blocks
+    uint64_t n_strk = 0, n_pay = 0, n_dens = 0, n_pm = 0;
+    hexo::SwiftTotals tot{0, 0, 0, 0, 0, 0, 0};
     const uint32_t rows = grads ? 6u : 1u;
     for (uint32_t i = 0; i < n; ++i) {
         const hexo_swift_request& q = reqs[i];
@@ -219,56 +236,110 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
         R.J = g.J;
         R.n_strikes = q.n_strikes;
         R.rows = rows;
+        R.log2_2J = 1u + (uint32_t)__builtin_ctz(g.J);
+        // one twiddle table per distinct J
+        size_t t = 0;
+        while (t < tables.size() && tables[t].J != g.J) ++t;
+        if (t == tables.size()) {
+            tables.push_back(hexo::SwiftTable{g.J, 0, tot.tw});
+            tot.tw += 4ull * g.J;
+        }
+        R.tw_off = tables[t].off;
         R.strike_off = n_strk;
         R.out_off = n_strk;
-        R.tw_off = n_tw;
         R.pay_off = n_pay;
         R.coeff_off = n_pay;
         R.dens_off = n_dens;
         R.pm_off = n_pm;
         const uint64_t nk = (uint64_t)((int64_t)g.k2 - (int64_t)g.k1 + 1);
+        const bool fft = g.J <= hexo::kSwiftFftMaxJ;
+        if (fft) {
+            fft_chains.push_back(i);
+            tot.fft_lds_bytes = std::max(tot.fft_lds_bytes, (uint32_t)(sizeof(hexo::Cplx) * 2 * g.J));
+        } else {
+            tot.dens += nk;
+            tot.dcoef += g.J;
+        }
         n_strk += q.n_strikes;
-        n_tw += 4ull * g.J;
         n_pay += g.J;
         n_dens += nk;
         n_pm += (uint64_t)rows * g.J;
-        tot.tw += 4ull * g.J;
-        tot.dens += nk;
         tot.j += g.J;
-        tot.strikes += q.n_strikes;
-        ends[i] = tot.tw;
+        tot.sblocks += (q.n_strikes + hexo::kSwiftStrikeBlock - 1) / hexo::kSwiftStrikeBlock;
+        ends[i] = tot.j;
         ends[n + i] = tot.dens;
-        ends[2 * (size_t)n + i] = tot.j;
-        ends[3 * (size_t)n + i] = tot.strikes;
+        ends[2 * (size_t)n + i] = tot.dcoef;
+        ends[3 * (size_t)n + i] = tot.sblocks;
     }
+    tot.n_fft = (uint32_t)fft_chains.size();
+    std::vector<uint64_t> tab_end(tables.size());
+    for (size_t t = 0; t < tables.size(); ++t) tab_end[t] = tables[t].off + 4ull * tables[t].J;
+    const uint32_t n_tab = (uint32_t)tables.size();
     auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t b_recs = align(sizeof(hexo::DevSwift) * n), b_ends = align(sizeof(uint64_t) * 4 * (size_t)n),
-                 b_strk = align(8 * n_strk), b_tw = align(16 * n_tw), b_pay = align(16 * n_pay),
-                 b_dens = align(8 * n_dens), b_coeff = align(16 * n_pay), b_pm = align(16 * n_pm),
-                 b_prices = align(8 * n_strk), b_grads = grads ? align(40 * n_strk) : 0;
-    const size_t in_bytes = b_recs + b_ends + b_strk;
+    const size_t b_recs = align(sizeof(hexo::DevSwift) * n), b_tabs = align(sizeof(hexo::SwiftTable) * n_tab),
+                 b_tabend = align(8 * (size_t)n_tab), b_fft = align(4 * fft_chains.size()),
+                 b_ends = align(sizeof(uint64_t) * 4 * (size_t)n), b_strk = align(8 * n_strk),
+                 b_tw = align(16 * tot.tw), b_pay = align(16 * n_pay), b_dens = align(8 * n_dens),
+                 b_coeff = align(16 * n_pay), b_pm = align(16 * n_pm), b_prices = align(8 * n_strk),
+                 b_grads = grads ? align(40 * n_strk) : 0;
+    const size_t in_bytes = b_recs + b_tabs + b_tabend + b_fft + b_ends + b_strk;
     const size_t total = in_bytes + b_tw + b_pay + b_dens + b_coeff + b_pm + b_prices + b_grads;
-    std::vector<unsigned char> host(in_bytes, 0);
-    std::memcpy(host.data(), recs.data(), sizeof(hexo::DevSwift) * n);
-    std::memcpy(host.data() + b_recs, ends.data(), sizeof(uint64_t) * 4 * (size_t)n);
-    {
-        double* ks = reinterpret_cast<double*>(host.data() + b_recs + b_ends);
-        for (uint32_t i = 0; i < n; ++i) std::memcpy(ks + recs[i].strike_off, reqs[i].strikes, 8 * reqs[i].n_strikes);
-    }
+    const size_t out_bytes = 8 * n_strk + (grads ? 40 * n_strk : 0);
     std::lock_guard<std::recursive_mutex> lock(ctx->mu);
     if (hipSetDevice(ctx->device) != hipSuccess) return ctx->set_error(HEXO_ERR_HIP, "hipSetDevice failed");
-    unsigned char* dev = nullptr;
-    if (hipMalloc(&dev, total) != hipSuccess) return ctx->set_error(HEXO_ERR_NOMEM, "hipMalloc failed");
+    if (!ctx->swift_host.reserve(std::max(in_bytes, out_bytes)))
+        return ctx->set_error(HEXO_ERR_NOMEM, "hipHostMalloc failed");
+    if (total > ctx->swift_dev_bytes) {
+        // the previous workspace may still be read by work queued on the stream
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->set_error(HEXO_ERR_HIP, "stream sync failed");
+        if (ctx->swift_dev) (void)hipFree(ctx->swift_dev);
+        ctx->swift_dev = nullptr;
+        ctx->swift_dev_bytes = 0;
+        const size_t want = total + total / 2;
+        if (hipMalloc(&ctx->swift_dev, want) != hipSuccess) {
+            ctx->swift_dev = nullptr;
+            return ctx->set_error(HEXO_ERR_NOMEM, "hipMalloc failed");
+        }
+        ctx->swift_dev_bytes = want;
+    }
+    unsigned char* host = reinterpret_cast<unsigned char*>(ctx->swift_host.p);
+    std::memset(host, 0, in_bytes);
+    size_t off = 0;
+    std::memcpy(host + off, recs.data(), sizeof(hexo::DevSwift) * n);
+    off += b_recs;
+    std::memcpy(host + off, tables.data(), sizeof(hexo::SwiftTable) * n_tab);
+    off += b_tabs;
+    std::memcpy(host + off, tab_end.data(), 8 * (size_t)n_tab);
+    off += b_tabend;
+    if (!fft_chains.empty()) std::memcpy(host + off, fft_chains.data(), 4 * fft_chains.size());
+    off += b_fft;
+    std::memcpy(host + off, ends.data(), sizeof(uint64_t) * 4 * (size_t)n);
+    off += b_ends;
+    {
+        double* ks = reinterpret_cast<double*>(host + off);
+        for (uint32_t i = 0; i < n; ++i) std::memcpy(ks + recs[i].strike_off, reqs[i].strikes, 8 * reqs[i].n_strikes);
+    }
+    unsigned char* dev = reinterpret_cast<unsigned char*>(ctx->swift_dev);
     hexo::SwiftArgs a{};
-    a.reqs = reinterpret_cast<const hexo::DevSwift*>(dev);
+    off = 0;
+    a.reqs = reinterpret_cast<const hexo::DevSwift*>(dev + off);
     a.n_req = n;
-    const uint64_t* d_ends = reinterpret_cast<const uint64_t*>(dev + b_recs);
-    a.tw_end = d_ends;
+    off += b_recs;
+    a.tables = reinterpret_cast<const hexo::SwiftTable*>(dev + off);
+    a.n_tables = n_tab;
+    off += b_tabs;
+    a.tab_end = reinterpret_cast<const uint64_t*>(dev + off);
+    off += b_tabend;
+    a.fft_chains = reinterpret_cast<const uint32_t*>(dev + off);
+    off += b_fft;
+    const uint64_t* d_ends = reinterpret_cast<const uint64_t*>(dev + off);
+    a.j_end = d_ends;
     a.dens_end = d_ends + n;
-    a.j_end = d_ends + 2 * (size_t)n;
-    a.strike_end = d_ends + 3 * (size_t)n;
-    a.strikes = reinterpret_cast<const double*>(dev + b_recs + b_ends);
-    size_t off = in_bytes;
+    a.dcoef_end = d_ends + 2 * (size_t)n;
+    a.sblk_end = d_ends + 3 * (size_t)n;
+    off += b_ends;
+    a.strikes = reinterpret_cast<const double*>(dev + off);
+    off += b_strk;
     a.tw = reinterpret_cast<hexo::Cplx*>(dev + off);
     off += b_tw;
     a.pay = reinterpret_cast<hexo::Cplx*>(dev + off);
@@ -283,13 +354,17 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
     off += b_prices;
     a.grads = grads ? reinterpret_cast<double*>(dev + off) : nullptr;
     hipStream_t st = ctx->stream;
-    bool ok = hipMemcpyAsync(dev, host.data(), in_bytes, hipMemcpyHostToDevice, st) == hipSuccess &&
-              hexo::launch_swift(a, tot, st) == hipSuccess &&
-              hipMemcpyAsync(prices, a.prices, 8 * n_strk, hipMemcpyDeviceToHost, st) == hipSuccess &&
-              (!grads || hipMemcpyAsync(grads, a.grads, 40 * n_strk, hipMemcpyDeviceToHost, st) == hipSuccess) &&
-              hipStreamSynchronize(st) == hipSuccess;
-    (void)hipFree(dev);
+    // inputs from the pinned stage; results back into it (the same buffer: the upload has
+    // completed when the stream reaches the result copies)
+    const bool ok = hipMemcpyAsync(dev, host, in_bytes, hipMemcpyHostToDevice, st) == hipSuccess &&
+                    hexo::launch_swift(a, tot, st) == hipSuccess &&
+                    hipMemcpyAsync(host, a.prices, 8 * n_strk, hipMemcpyDeviceToHost, st) == hipSuccess &&
+                    (!grads || hipMemcpyAsync(host + 8 * n_strk, a.grads, 40 * n_strk, hipMemcpyDeviceToHost, st) ==
+                                   hipSuccess) &&
+                    hipStreamSynchronize(st) == hipSuccess;
     if (!ok) return ctx->set_error(HEXO_ERR_HIP, "SWIFT launch failed");
+    std::memcpy(prices, host, 8 * n_strk);
+    if (grads) std::memcpy(grads, host + 8 * n_strk, 40 * n_strk);
     if (grids_used) std::memcpy(grids_used, grids.data(), sizeof(hexo_swift_grid) * n);
     return HEXO_OK;
 }

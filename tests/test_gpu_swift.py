@@ -103,3 +103,22 @@ def test_mc_european_prices_agree_with_swift(ctx):
         dev = np.abs(euro - q)
         say("  T", r.expiries[0], "max |mc - swift| / se", (dev / se).max())
         assert np.all(dev < 4 * se + 0.02)
+
+
+def test_swift_fft_and_direct_sum_paths(ctx, oracle):
+    """Chains at both sides of the LDS-FFT size limit (J <= 4096: radix-2 FFTs in one workgroup;
+    larger J: direct sums), and a tiny grid, against the oracle's direct DFTs."""
+    say("gpu swift fft / direct paths")
+    base = H.SwiftChain(0.04, 0.04, -0.7, 2.0, 0.3, S0=100.0, T=1.0, strikes=[80.0, 100.0, 120.0], r=0.02)
+    g = list(H.swift_grid(base))
+    kw = dict(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0, T=1.0, strikes=[80.0, 100.0, 120.0],
+              r=0.02)
+    chains = [H.SwiftChain(**kw, grid=g[:7] + [J]) for J in (4096, 8192, 16384)]
+    chains.append(H.SwiftChain(**kw, grid=[1, 2, 2 ** 0.5, -1.0, 1.0, -2, 2, 2]))
+    res = ctx.swift_price(chains, grads=True)
+    for ch, (p, gr, used) in zip(chains, res):
+        op, _, og = oracle.swift_price([ch.v0, ch.theta, ch.rho, ch.kappa, ch.sigma], ch.S0, ch.r, ch.T, ch.strikes,
+                                       swift_params=ch.grid, grads=True)
+        say("  J", used[7], "max |gpu - oracle|", np.abs(p - op).max())
+        np.testing.assert_allclose(p, op, rtol=1e-10, atol=1e-9)
+        np.testing.assert_allclose(gr, og, rtol=1e-9, atol=1e-9)
