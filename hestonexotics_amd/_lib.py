@@ -75,6 +75,25 @@ class hexo_swift_request(C.Structure):
                 ("trunc", C.c_double), ("grid", hexo_swift_grid)]
 
 
+class hexo_calib_chain(C.Structure):
+    _fields_ = [("T", C.c_double), ("n_options", C.c_uint32), ("reserved", C.c_uint32),
+                ("strikes", C.POINTER(C.c_double)), ("prices", C.POINTER(C.c_double)),
+                ("volumes", C.POINTER(C.c_double))]
+
+
+class hexo_calib_config(C.Structure):
+    _fields_ = [("max_iter", C.c_uint32), ("flags", C.c_uint32), ("initial", C.c_double * 5),
+                ("mu_scale", C.c_double), ("eps_grad", C.c_double), ("eps_step", C.c_double),
+                ("eps_err", C.c_double), ("r", C.c_double), ("trunc", C.c_double)]
+
+
+class hexo_calib_result(C.Structure):
+    _fields_ = [("params", C.c_double * 5), ("initial", C.c_double * 5), ("err0", C.c_double), ("err", C.c_double),
+                ("iterations", C.c_uint32), ("reason", C.c_uint32), ("n_price_evals", C.c_uint32),
+                ("n_jacobian_evals", C.c_uint32)]
+
+
+CALIB_INITIAL_GIVEN = 1
 SWIFT_FIXED_GRID = 1
 SWIFT_MAX_J = 16384
 
@@ -122,6 +141,9 @@ SIGNATURES = {
     "hexo_server_run_trace": (C.c_int, [_P, _req, _d, C.c_uint32, _d, _d, C.POINTER(hexo_ticket_info)]),
     "hexo_inject_fault": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
     "hexo_swift_grid_for": (C.c_int, [C.POINTER(hexo_swift_request), C.POINTER(hexo_swift_grid)]),
+    "hexo_calib_defaults": (C.c_int, [C.POINTER(hexo_calib_config)]),
+    "hexo_calibrate": (C.c_int, [_P, C.c_double, C.POINTER(hexo_calib_chain), C.c_uint32,
+                                 C.POINTER(hexo_calib_config), C.POINTER(hexo_calib_result)]),
     "hexo_swift_price": (C.c_int, [_P, C.POINTER(hexo_swift_request), C.c_uint32, _d, _d,
                                    C.POINTER(hexo_swift_grid)]),
 }

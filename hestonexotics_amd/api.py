@@ -16,9 +16,10 @@ from typing import Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import (ASIAN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64, SWIFT_FIXED_GRID, TRAPEZOID,
-                   HexoError, hexo_batch_stats, hexo_request, hexo_server_config, hexo_server_stats,
-                   hexo_swift_grid, hexo_swift_request, hexo_ticket_info, hexo_tuning)
+from ._lib import (ASIAN, CALIB_INITIAL_GIVEN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64,
+                   SWIFT_FIXED_GRID, TRAPEZOID, HexoError, hexo_batch_stats, hexo_calib_chain, hexo_calib_config,
+                   hexo_calib_result, hexo_request, hexo_server_config, hexo_server_stats, hexo_swift_grid,
+                   hexo_swift_request, hexo_ticket_info, hexo_tuning)
 
 __all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "DRIFT", "DISCOUNT", "TRAPEZOID", "FP64", "FP32",
            "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "SwiftChain", "SwiftCall", "swift_grid"]
@@ -321,6 +322,42 @@ class Context:
         v0, theta, rho, kappa, sigma). Returns a list of (prices, grads or None, grid) per chain."""
         return SwiftCall(self, chains, grads).run().split()
 
+
+    # ---- calibration ----
+    def calibrate(self, S0: float, chains, initial=None, **config) -> dict:
+        """Heston calibration to market prices (hexo_calibrate). chains: sequence of
+        (T, strikes, prices[, volumes]). initial: None (the reference's starting point from the
+        average implied volatility) or (v0, theta, rho, kappa, sigma). config: max_iter,
+        mu_scale, eps_grad, eps_step, eps_err, r, trunc (defaults: the reference's)."""
+        arr = (hexo_calib_chain * len(chains))()
+        keep = []
+        for i, ch in enumerate(chains):
+            T, K, P = ch[0], ch[1], ch[2]
+            V = ch[3] if len(ch) > 3 else None
+            k = np.ascontiguousarray(K, np.float64)
+            p = np.ascontiguousarray(P, np.float64)
+            keep += [k, p]
+            arr[i].T = float(T)
+            arr[i].n_options = len(k)
+            arr[i].strikes = _dp(k)
+            arr[i].prices = _dp(p)
+            if V is not None:
+                v = np.ascontiguousarray(V, np.float64)
+                keep.append(v)
+                arr[i].volumes = _dp(v)
+        cfg = hexo_calib_config()
+        self.lib.hexo_calib_defaults(C.byref(cfg))
+        for key, val in config.items():
+            setattr(cfg, key, val)
+        if initial is not None:
+            cfg.flags |= CALIB_INITIAL_GIVEN
+            for j in range(5):
+                cfg.initial[j] = float(initial[j])
+        res = hexo_calib_result()
+        self._check(self.lib.hexo_calibrate(self.h, float(S0), arr, len(chains), C.byref(cfg), C.byref(res)))
+        return {"params": np.array(res.params[:]), "initial": np.array(res.initial[:]), "err0": res.err0,
+                "err": res.err, "iterations": res.iterations, "reason": res.reason,
+                "nfev": res.n_price_evals, "njev": res.n_jacobian_evals}
 
     # ---- tuning / measurement ----
     def get_tuning(self) -> dict:

@@ -260,6 +260,56 @@ int hexo_swift_grid_for(const hexo_swift_request* req, hexo_swift_grid* out);
 int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, uint32_t n, double* prices, double* grads,
                      hexo_swift_grid* grids_used);
 
+/* ---- calibration ----
+ * Replaces calibrate() (src/HCalibration.cpp:62-117): Levenberg-Marquardt over SWIFT prices and
+ * the SWIFT Jacobian of every chain (get_prices_for_levmar / get_jacobian_for_levmar,
+ * src/HCalibration.cpp:37-59, levmar 2.6 dlevmar_der), starting from the volume-weighted average
+ * Black-Scholes implied volatility (src/BSM.cpp:51-66): v0 = theta = iv^2, rho = -0.2,
+ * kappa = sigma = 1. Every price and Jacobian evaluation is one hexo_swift_price call over all
+ * chains. */
+typedef struct hexo_calib_chain {   /* options_chain (src/inc/Types.h) */
+    double T;                        /* time to expiry in years */
+    uint32_t n_options;
+    uint32_t reserved;               /* must be 0 */
+    const double* strikes;
+    const double* prices;            /* market prices (option::price) */
+    const double* volumes;           /* weights of the initial implied-volatility average
+                                        (option::volume); NULL: all 1 */
+} hexo_calib_chain;
+
+#define HEXO_CALIB_INITIAL_GIVEN 1u   /* start from hexo_calib_config.initial */
+
+typedef struct hexo_calib_config {
+    uint32_t max_iter;               /* 100 in the reference */
+    uint32_t flags;                  /* HEXO_CALIB_INITIAL_GIVEN or 0 */
+    double initial[5];               /* v0, theta, rho, kappa, sigma */
+    double mu_scale;                 /* initial damping / max diag(J^T J): LM_INIT_MU = 1e-3 */
+    double eps_grad;                 /* stop when max |J^T e| <= eps_grad (1e-10) */
+    double eps_step;                 /* stop when |dp| <= eps_step |p| (1e-10) */
+    double eps_err;                  /* stop when |e|^2 <= eps_err (1e-10) */
+    double r;                        /* risk-free rate of the model (0.0005, src/inc/BSM.h:10) */
+    double trunc;                    /* SWIFT truncation precision (1e-10) */
+} hexo_calib_config;
+
+typedef struct hexo_calib_result {
+    double params[5];                /* v0, theta, rho, kappa, sigma */
+    double initial[5];               /* the starting point used */
+    double err0, err;                /* |e|^2 at the start and at the end */
+    uint32_t iterations;
+    uint32_t reason;                 /* levmar's stop codes: 1 small gradient, 2 small step,
+                                        3 iteration limit, 4 singular system, 5 no further
+                                        reduction, 6 small error, 7 invalid values. The
+                                        reference accepts 2 and 6. */
+    uint32_t n_price_evals, n_jacobian_evals;
+} hexo_calib_result;
+
+/* Fills a config with the reference's settings. */
+int hexo_calib_defaults(hexo_calib_config* cfg);
+/* Calibrates the five Heston parameters to the market prices. cfg NULL: the reference's
+ * settings. Returns HEXO_OK when the iteration ran; out->reason says how it stopped. */
+int hexo_calibrate(hexo_ctx* ctx, double S0, const hexo_calib_chain* chains, uint32_t n_chains,
+                   const hexo_calib_config* cfg, hexo_calib_result* out);
+
 #ifdef __cplusplus
 }
 #endif
