@@ -1,12 +1,16 @@
 // hexo_cli.cpp -- the `hexo` command line for the pricing path.
 //
-// Keeps the reference's grammar for the Asian pricing entry (src/Main.cpp:75-96, src/inc/Main.h:12):
+// Keeps the reference's grammar (src/Main.cpp:58-96, src/inc/Main.h:12):
 //   hexo -p asian all <underlying> [<volume type> <# volume>]   prices every call of every chain
+//   hexo -c <underlying> [<volume type> <# volume>]             calibrates the Heston parameters
 //   hexo -t rng | -t simulation                                  self-tests on the GPU
 //   hexo -h
-// The reference downloads quotes and chains over HTTPS and reads calibrated parameters from
-// sqlite (src/Main.cpp:77-86); this build reads both from an offline chain file
-// $HEXO_DATA_DIR/<underlying>.chains (default ./data), one record per line:
+// The reference downloads quotes and chains over HTTPS and keeps calibrated parameters in
+// sqlite (src/Main.cpp:64-86); this build reads quotes and chains from an offline chain file
+// $HEXO_DATA_DIR/<underlying>.chains (default ./data) and keeps calibrated parameters in
+// $HEXO_DATA_DIR/<underlying>.params (one line: v0 theta rho kappa sigma). As in the reference,
+// -p uses stored parameters (a `params` line of the chain file, else the .params file) and
+// calibrates and stores them when there are none. Chain file, one record per line:
 //   quote <bid>
 //   params <v0> <theta> <rho> <kappa> <sigma>          (HParams order)
 //   expiry <days to expiry>
@@ -27,12 +31,14 @@
 #include <string>
 #include <vector>
 
+#include "../../include/hexo/HCalibration.h"
 #include "../../include/hexo/HSimulation.h"
 
 namespace {
 
 const char kHelp[] =
-    "hexo [{-p asian all <underlying> [<volume type> <# volume>]}...] [{-t rng|simulation}...] [-h]";
+    "hexo [{-p asian all <underlying> [<volume type> <# volume>]}...] [{-c <underlying> [<volume type> <# volume>]}...] "
+    "[{-t rng|simulation}...] [-h]";
 const double kTradingDays = (double)(5 * (365 / 7) + 365 % 7);
 const double kExpLowerBound = 0.1;
 const double kYearlyRiskFree = 0.0005;
@@ -90,6 +96,34 @@ UnderlyingData read_chains(const std::string& sym, const std::string& vol_type, 
     return d;
 }
 
+std::string data_path(const std::string& sym, const char* ext) {
+    const char* dir = std::getenv("HEXO_DATA_DIR");
+    return std::string(dir ? dir : "data") + "/" + sym + ext;
+}
+
+// The offline stand-in for ParamsDB::fetch / insertupdate (src/inc/DB.h).
+bool load_params(const std::string& sym, HParams& p) {
+    std::ifstream in(data_path(sym, ".params"));
+    return in && (in >> p.v_0 >> p.v_m >> p.rho >> p.kappa >> p.sigma);
+}
+void store_params(const std::string& sym, const HParams& p) {
+    std::ofstream out(data_path(sym, ".params"));
+    out << std::setprecision(17) << p.v_0 << ' ' << p.v_m << ' ' << p.rho << ' ' << p.kappa << ' ' << p.sigma << '\n';
+    if (!out) throw std::runtime_error("cannot write " + data_path(sym, ".params"));
+}
+
+// calibrate() with the reference's progress lines (src/Main.cpp:69-72, src/HCalibration.cpp:81,109).
+HParams calibrate_and_store(const std::string& sym, const UnderlyingData& d) {
+    unsigned n = 0;
+    for (const auto& c : d.chains) n += (unsigned)c.options.size();
+    std::cout << "setup completed\t# observations: " << n << '\n';
+    const HParams p = calibrate(d.S, d.chains);
+    std::cout << "v_0: " << p.v_0 << "\tv_m: " << p.v_m << "\trho: " << p.rho << "\tkappa: " << p.kappa
+              << "\tsigma: " << p.sigma << '\n';
+    store_params(sym, p);
+    return p;
+}
+
 double norm_cdf(double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); }
 
 double bs_call(double S, double K, double vol, double T) {
@@ -130,8 +164,7 @@ std::vector<ffloat> run_price(const UnderlyingData& d, unsigned paths, unsigned 
 
 int price_all(const std::string& sym, const std::string& vol_type, long vol_n) {
     UnderlyingData d = read_chains(sym, vol_type, vol_n);
-    if (!d.have_params)
-        throw std::runtime_error("no calibrated parameters for " + sym + " (calibration is not part of this build)");
+    if (!d.have_params && !load_params(sym, d.p)) d.p = calibrate_and_store(sym, d);
     unsigned n_opts = 0;
     for (const auto& c : d.chains) n_opts += (unsigned)c.options.size();
     std::cout << "params, v0: " << d.p.v_0 << "\tv_m: " << d.p.v_m << "\trho: " << d.p.rho << "\tkappa: " << d.p.kappa
@@ -228,6 +261,20 @@ int main(int argc, char* argv[]) {
                     cur += 2;
                 }
                 if (price_all(sym, vol_type, vol_n)) return 1;
+            } else if (a == "-c" || a == "--calibrate") {
+                if (cur + 1 >= argc) throw std::runtime_error(kHelp);
+                const std::string sym = argv[cur + 1];
+                std::string vol_type = "open_interest";
+                long vol_n = 0;
+                cur += 2;
+                if (cur + 1 < argc && argv[cur][0] != '-') {
+                    vol_type = argv[cur];
+                    vol_n = std::stol(argv[cur + 1]);
+                    cur += 2;
+                }
+                const UnderlyingData d = read_chains(sym, vol_type, vol_n);
+                std::cout << "Options data downloaded and parsed\n";
+                calibrate_and_store(sym, d);
             } else if (a == "-t" || a == "--test") {
                 if (cur + 1 >= argc) throw std::runtime_error(kHelp);
                 const std::string t = argv[cur + 1];

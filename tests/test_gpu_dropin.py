@@ -50,3 +50,29 @@ def test_cli_self_tests():
 def test_cli_rejects_bad_grammar():
     assert run_cli("-p", "european", "all", "DEMO").returncode == 2
     assert run_cli("-q").returncode == 2
+
+
+def test_cli_calibrates_and_stores_parameters(tmp_path):
+    """`hexo -c <SYM>` (src/Main.cpp:64-72) on a chain file whose asks are SWIFT prices of known
+    parameters: the stored parameters recover them; `hexo -p asian all` then uses them."""
+    print("gpu cli -c SYNTH", flush=True)
+    from oracle import oracle as O
+    p_true = [0.04, 0.05, -0.6, 1.8, 0.5]
+    lines = ["quote 100.0"]
+    for days in (65, 130, 261, 522):
+        T = days / 261.0
+        K = [80.0, 85.0, 90.0, 95.0, 100.0, 105.0, 110.0, 115.0, 120.0]
+        prices, _ = O.swift_price(p_true, 100.0, 0.0005, T, K, trunc=1e-10)
+        lines.append(f"expiry {days}")
+        lines += [f"call {k} {p:.12f} {p:.12f} 100 100" for k, p in zip(K, prices)]
+    (tmp_path / "SYNTH.chains").write_text("\n".join(lines) + "\n")
+    env = {"HEXO_DATA_DIR": str(tmp_path)}
+    out = subprocess.run([CLI, "-c", "SYNTH"], capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, **env))
+    assert out.returncode == 0, out.stdout + out.stderr
+    stored = [float(v) for v in (tmp_path / "SYNTH.params").read_text().split()]
+    assert np.abs(np.array(stored) - np.array(p_true)).max() < 1e-3, stored
+    out = subprocess.run([CLI, "-p", "asian", "all", "SYNTH"], capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, HEXO_PATHS="2000", HEXO_STEPS="50", **env))
+    assert out.returncode == 0, out.stderr
+    assert len([l for l in out.stdout.splitlines() if l.startswith("S:")]) == 36
