@@ -164,6 +164,35 @@ def swift_leg(ctx, n_sets: int, reps: int = 5):
                                                        f"chains in {cpu_s:.2f} s"}}
 
 
+def calibration_leg(ctx):
+    """The reference's calibration (hexo_calibrate) on a synthetic market: 8 maturities x 21 strikes
+    of SWIFT prices of known parameters, from the reference's implied-volatility start; the
+    oracle's restatement (one CPU thread, oracle SWIFT) on the same problem as the CPU baseline."""
+    from oracle import calib as CB
+    from oracle import oracle as O
+    p_true = [0.04, 0.05, -0.6, 1.8, 0.5]
+    chains = []
+    for T in (0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 4.0):
+        K = list(np.linspace(70.0, 130.0, 21))
+        prices, _ = O.swift_price(p_true, 100.0, CB.YEARLY_RISK_FREE, T, K, trunc=1e-10)
+        chains.append((T, K, prices, [1.0] * len(K)))
+    ctx.calibrate(100.0, chains)
+    t0 = time.perf_counter()
+    res = ctx.calibrate(100.0, chains)
+    gpu_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    ref = CB.levenberg_marquardt(100.0, chains, CB.initial_point(100.0, chains))
+    cpu_s = time.perf_counter() - t1
+    return {"workload": "Heston calibration to 8 maturities x 21 strikes of SWIFT prices of known parameters, "
+                        "from the implied-volatility start", "ms_per_calibration": gpu_s * 1e3,
+            "iterations": res["iterations"], "reason": res["reason"], "price_evals": res["nfev"],
+            "jacobian_evals": res["njev"], "max_param_err": float(np.max(np.abs(res["params"] - p_true))),
+            "max_param_diff_vs_oracle": float(np.max(np.abs(res["params"] - ref["params"]))),
+            "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "calibrations/s", "cores": 1, "kind": "port",
+                             "sample": f"oracle Levenberg-Marquardt over oracle SWIFT, one calibration in "
+                                       f"{cpu_s:.2f} s"}}
+
+
 def time_batch(ctx, batch, steps: int, warmup: int):
     for _ in range(warmup):
         batch.run_eager()
@@ -374,6 +403,8 @@ def main():
     if not args.no_swift:
         out["swift"] = swift_leg(ctx, args.swift_params)
         log("SWIFT:", out["swift"])
+        out["calibration"] = calibration_leg(ctx)
+        log("calibration:", out["calibration"])
 
     if args.serve_seconds > 0:
         reqs4, arr = W.c4(duration_s=args.serve_seconds, precision=H.FP32)
