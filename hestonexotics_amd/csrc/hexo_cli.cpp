@@ -3,14 +3,16 @@
 // Keeps the reference's grammar (src/Main.cpp:58-96, src/inc/Main.h:12):
 //   hexo -p asian all <underlying> [<volume type> <# volume>]   prices every call of every chain
 //   hexo -c <underlying> [<volume type> <# volume>]             calibrates the Heston parameters
+//   hexo -d <underlying> [<volume type> <# volume>]             lists the market data (no GPU)
 //   hexo -t rng | -t simulation                                  self-tests on the GPU
 //   hexo -h
 // The reference downloads quotes and chains over HTTPS and keeps calibrated parameters in
-// sqlite (src/Main.cpp:64-86); this build reads quotes and chains from an offline chain file
-// $HEXO_DATA_DIR/<underlying>.chains (default ./data) and keeps calibrated parameters in
-// $HEXO_DATA_DIR/<underlying>.params (one line: v0 theta rho kappa sigma). As in the reference,
-// -p uses stored parameters (a `params` line of the chain file, else the .params file) and
-// calibrates and stores them when there are none. Chain file, one record per line:
+// sqlite (src/Main.cpp:64-86). This build reads them offline from $HEXO_DATA_DIR (default ./data):
+// the Tradier JSON responses saved under <underlying>/ (market_data.hpp), or a compact chain file
+// <underlying>.chains; parameters live in HParams.db there (the reference's table) or, without
+// libsqlite3, in <underlying>.params. As in the reference, -p uses stored parameters (a `params`
+// line of the chain file, else the store) and calibrates and stores them when there are none.
+// Chain file, one record per line:
 //   quote <bid>
 //   params <v0> <theta> <rho> <kappa> <sigma>          (HParams order)
 //   expiry <days to expiry>
@@ -33,12 +35,13 @@
 
 #include "../../include/hexo/HCalibration.h"
 #include "../../include/hexo/HSimulation.h"
+#include "market_data.hpp"
 
 namespace {
 
 const char kHelp[] =
-    "hexo [{-p asian all <underlying> [<volume type> <# volume>]}...] [{-c <underlying> [<volume type> <# volume>]}...] "
-    "[{-t rng|simulation}...] [-h]";
+    "hexo [{-c <underlying> [<volume type> <# volume>]}...] [{-d <underlying> [<volume type> <# volume>]}...] "
+    "[{-p asian all <underlying> [<volume type> <# volume>]}...] [{-t rng|simulation}...] [-h]";
 const double kTradingDays = (double)(5 * (365 / 7) + 365 % 7);
 const double kExpLowerBound = 0.1;
 const double kYearlyRiskFree = 0.0005;
@@ -50,11 +53,23 @@ struct UnderlyingData {
     std::list<options_chain> chains;
 };
 
-UnderlyingData read_chains(const std::string& sym, const std::string& vol_type, long vol_n) {
+std::string data_dir() {
     const char* dir = std::getenv("HEXO_DATA_DIR");
-    const std::string path = std::string(dir ? dir : "data") + "/" + sym + ".chains";
+    return dir ? dir : "data";
+}
+
+UnderlyingData read_chains(const std::string& sym, const std::string& vol_type, long vol_n) {
+    const std::string path = data_dir() + "/" + sym + ".chains";
     std::ifstream in(path);
-    if (!in) throw std::runtime_error("cannot open " + path);
+    if (!in) {
+        // the API's JSON responses saved under <underlying>/
+        const char* today = std::getenv("HEXO_TODAY");
+        hexo_cli::MarketData md = hexo_cli::read_tradier_dir(data_dir() + "/" + sym, vol_type, vol_n, today ? today : "");
+        UnderlyingData d;
+        d.S = md.S;
+        d.chains = std::move(md.chains);
+        return d;
+    }
     UnderlyingData d;
     std::string line;
     bool keep = false;
@@ -96,22 +111,6 @@ UnderlyingData read_chains(const std::string& sym, const std::string& vol_type, 
     return d;
 }
 
-std::string data_path(const std::string& sym, const char* ext) {
-    const char* dir = std::getenv("HEXO_DATA_DIR");
-    return std::string(dir ? dir : "data") + "/" + sym + ext;
-}
-
-// The offline stand-in for ParamsDB::fetch / insertupdate (src/inc/DB.h).
-bool load_params(const std::string& sym, HParams& p) {
-    std::ifstream in(data_path(sym, ".params"));
-    return in && (in >> p.v_0 >> p.v_m >> p.rho >> p.kappa >> p.sigma);
-}
-void store_params(const std::string& sym, const HParams& p) {
-    std::ofstream out(data_path(sym, ".params"));
-    out << std::setprecision(17) << p.v_0 << ' ' << p.v_m << ' ' << p.rho << ' ' << p.kappa << ' ' << p.sigma << '\n';
-    if (!out) throw std::runtime_error("cannot write " + data_path(sym, ".params"));
-}
-
 // calibrate() with the reference's progress lines (src/Main.cpp:69-72, src/HCalibration.cpp:81,109).
 HParams calibrate_and_store(const std::string& sym, const UnderlyingData& d) {
     unsigned n = 0;
@@ -120,7 +119,7 @@ HParams calibrate_and_store(const std::string& sym, const UnderlyingData& d) {
     const HParams p = calibrate(d.S, d.chains);
     std::cout << "v_0: " << p.v_0 << "\tv_m: " << p.v_m << "\trho: " << p.rho << "\tkappa: " << p.kappa
               << "\tsigma: " << p.sigma << '\n';
-    store_params(sym, p);
+    hexo_cli::store_params(data_dir(), sym, p);
     return p;
 }
 
@@ -152,6 +151,17 @@ double imp_vol(double S, const option& o, double T) {
     return mid;
 }
 
+// The DOWNLOAD listing of the reference (src/Main.cpp:97-103).
+void list_options(const UnderlyingData& d) {
+    for (const auto& c : d.chains)
+        for (const option& o : c.options)
+            std::cout << "S: " << std::setw(10) << std::right << std::setfill(' ') << std::fixed << std::setprecision(2)
+                      << d.S << "\tstrike: " << o.strike << "\tbid: " << o.bid << "\task: " << o.price
+                      << "\tvolume: " << o.volume << "\timp vol: " << imp_vol(d.S, o, c.time_to_expiry)
+                      << "\tlb: " << d.S - std::exp(-kYearlyRiskFree * c.time_to_expiry) * o.strike
+                      << "\texpiry time: " << c.time_to_expiry * kTradingDays << '\n';
+}
+
 unsigned env_u(const char* name, unsigned dflt) {
     const char* v = std::getenv(name);
     return v ? (unsigned)std::strtod(v, nullptr) : dflt;
@@ -164,7 +174,7 @@ std::vector<ffloat> run_price(const UnderlyingData& d, unsigned paths, unsigned 
 
 int price_all(const std::string& sym, const std::string& vol_type, long vol_n) {
     UnderlyingData d = read_chains(sym, vol_type, vol_n);
-    if (!d.have_params && !load_params(sym, d.p)) d.p = calibrate_and_store(sym, d);
+    if (!d.have_params && !hexo_cli::fetch_params(data_dir(), sym, d.p)) d.p = calibrate_and_store(sym, d);
     unsigned n_opts = 0;
     for (const auto& c : d.chains) n_opts += (unsigned)c.options.size();
     std::cout << "params, v0: " << d.p.v_0 << "\tv_m: " << d.p.v_m << "\trho: " << d.p.rho << "\tkappa: " << d.p.kappa
@@ -261,7 +271,7 @@ int main(int argc, char* argv[]) {
                     cur += 2;
                 }
                 if (price_all(sym, vol_type, vol_n)) return 1;
-            } else if (a == "-c" || a == "--calibrate") {
+            } else if (a == "-c" || a == "--calibrate" || a == "-d" || a == "--download") {
                 if (cur + 1 >= argc) throw std::runtime_error(kHelp);
                 const std::string sym = argv[cur + 1];
                 std::string vol_type = "open_interest";
@@ -273,8 +283,12 @@ int main(int argc, char* argv[]) {
                     cur += 2;
                 }
                 const UnderlyingData d = read_chains(sym, vol_type, vol_n);
-                std::cout << "Options data downloaded and parsed\n";
-                calibrate_and_store(sym, d);
+                if (a == "-d" || a == "--download") {
+                    list_options(d);
+                } else {
+                    std::cout << "Options data downloaded and parsed\n";
+                    calibrate_and_store(sym, d);
+                }
             } else if (a == "-t" || a == "--test") {
                 if (cur + 1 >= argc) throw std::runtime_error(kHelp);
                 const std::string t = argv[cur + 1];

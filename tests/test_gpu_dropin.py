@@ -70,9 +70,17 @@ def test_cli_calibrates_and_stores_parameters(tmp_path):
     out = subprocess.run([CLI, "-c", "SYNTH"], capture_output=True, text=True, timeout=120,
                          env=dict(os.environ, **env))
     assert out.returncode == 0, out.stdout + out.stderr
-    stored = [float(v) for v in (tmp_path / "SYNTH.params").read_text().split()]
+    if (tmp_path / "HParams.db").exists():  # the reference's table (src/inc/DB.h:31), via libsqlite3
+        import sqlite3
+        con = sqlite3.connect(str(tmp_path / "HParams.db"))
+        row = con.execute("SELECT V0, Vm, Rho, Kappa, Sigma FROM HestonParameters WHERE Name='SYNTH'").fetchone()
+        con.close()
+        stored = list(row)
+    else:
+        stored = [float(v) for v in (tmp_path / "SYNTH.params").read_text().split()]
     assert np.abs(np.array(stored) - np.array(p_true)).max() < 1e-3, stored
     out = subprocess.run([CLI, "-p", "asian", "all", "SYNTH"], capture_output=True, text=True, timeout=120,
                          env=dict(os.environ, HEXO_PATHS="2000", HEXO_STEPS="50", **env))
     assert out.returncode == 0, out.stderr
     assert len([l for l in out.stdout.splitlines() if l.startswith("S:")]) == 36
+    assert "setup completed" not in out.stdout  # the stored parameters were used, no recalibration
