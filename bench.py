@@ -237,7 +237,8 @@ def main():
     ap.add_argument("--no-swift", action="store_true")
     ap.add_argument("--swift-params", type=int, default=64, help="parameter sets of the SWIFT calibration batch")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape instead of running the autotuner on the workload")
     ap.add_argument("--profile", action="store_true",
                     help="profiling driver: --profile-passes eager FP32 passes of the workload, then a plain copy of "
                          "--copy-mib MiB (the known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE)")
@@ -296,8 +297,10 @@ def main():
         return
     reqs32 = W.c2(n_paths=args.paths, precision=H.FP32)
     nominal = W.path_steps(reqs32)
-    if args.autotune:
-        log("autotune:", ctx.autotune(W.c2(n_paths=max(args.paths // 8, 64), precision=H.FP32), reps=2))
+    if not args.no_autotune:
+        # the project's autotuner (hexo_autotune) picks paths per thread, block size and occupancy
+        # cap on the workload itself and leaves the winner set on the context
+        log("autotune:", ctx.autotune(W.c2(n_paths=args.paths, precision=H.FP32), reps=3))
     tuning = ctx.get_tuning()
 
     b32 = ctx.batch(reqs32)

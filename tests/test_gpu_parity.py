@@ -274,3 +274,19 @@ def test_paths_window_and_offset(ctx, oracle):
     shifted.n_paths, shifted.path_offset = 300, 128
     g2 = ctx.paths(shifted, 0, 300)
     np.testing.assert_allclose(g2, o[128:428], rtol=1e-12)
+
+
+def test_autotuner_picks_a_valid_shape_and_keeps_results(ctx):
+    say("gpu autotune")
+    reqs = W.c2(n_paths=20_000, maturities=[0.5, 2.0], n_strikes=8)
+    base = ctx.get_tuning()
+    ref = ctx.price_batch(reqs)
+    try:
+        best = ctx.autotune(reqs, reps=1)
+        assert best["paths_per_thread"] in (1, 2, 4) and best["block_threads"] in (64, 128, 256)
+        assert ctx.get_tuning() == best
+        for (p, s), (q, t) in zip(ref, ctx.price_batch(reqs)):
+            np.testing.assert_array_equal(p, q)
+            np.testing.assert_array_equal(s, t)
+    finally:
+        ctx.set_tuning(**base)
