@@ -243,6 +243,8 @@ def main():
                     help="profiling driver: --profile-passes eager FP32 passes of the workload, then a plain copy of "
                          "--copy-mib MiB (the known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--profile-passes", type=int, default=20)
+    ap.add_argument("--profile-workload", choices=("c2", "c3", "c5"), default="c2",
+                    help="workload of --profile: the grid (c2), the 512 uneven requests (c3), the stiff request (c5)")
     ap.add_argument("--profile-swift", action="store_true",
                     help="profiling driver for the SWIFT pricer: --profile-passes calls on the calibration batch")
     ap.add_argument("--copy-mib", type=int, default=512)
@@ -261,7 +263,10 @@ def main():
     info = ctx.device_info()
     log("device", info)
     if args.profile:
-        bp = ctx.batch(W.c2(n_paths=args.paths, precision=H.FP32))
+        prof_reqs = {"c2": lambda: W.c2(n_paths=args.paths, precision=H.FP32),
+                     "c3": lambda: W.c3(precision=H.FP32),
+                     "c5": lambda: [W.c5(precision=H.FP32)]}[args.profile_workload]()
+        bp = ctx.batch(prof_reqs)
         times = []
         for i in range(args.profile_passes):
             bp.run_eager()
