@@ -236,9 +236,15 @@ extern "C" int hexo_calibrate(hexo_ctx* ctx, double S0, const hexo_calib_chain* 
                     stop = 4;
                     break;
                 }
-                // trial points outside the Heston domain are rejected steps
+                // trial points outside the Heston domain, or where the pricer rejects the model
+                // (no SWIFT grid within its limits), are rejected steps
+                bool priced = false;
                 if (in_domain(pn)) {
-                    if ((rc = M.eval(pn, hn.data(), nullptr))) return rc;
+                    rc = M.eval(pn, hn.data(), nullptr);
+                    if (rc != HEXO_OK && rc != HEXO_ERR_INVALID) return rc;
+                    priced = rc == HEXO_OK;
+                }
+                if (priced) {
                     ++nfev;
                     double enL2 = 0.0;
                     for (size_t i = 0; i < n; ++i) {

@@ -13,7 +13,8 @@ with levmar's stopping rules) is restated here: damping mu = tau max diag(J^T J)
 iterate; a step solves (J^T J + mu I) dp = J^T e; it is taken when the gain ratio is positive,
 with mu *= max(1/3, 1 - (2 rho - 1)^3) and nu = 2, else mu *= nu, nu *= 2. Stop codes:
 1 small gradient, 2 small step, 3 iteration limit, 4 singular, 5 no further reduction,
-6 small error, 7 invalid values. Trial points outside the Heston domain count as rejected steps.
+6 small error, 7 invalid values. Trial points outside the Heston domain, or whose SWIFT grid
+exceeds the pricer's limit, count as rejected steps.
 """
 from __future__ import annotations
 
@@ -81,10 +82,22 @@ def valid(p):
     return p[0] > 0 and p[1] > 0 and p[3] > 0 and p[4] > 0 and -1 < p[2] < 1
 
 
+SWIFT_MAX_J = 16384  # largest SWIFT grid the pricer accepts (include/hexo.h HEXO_SWIFT_MAX_J)
+
+
+class Unpriceable(ValueError):
+    """The SWIFT grid for a parameter set exceeds SWIFT_MAX_J."""
+
+
 def model(p, S, chains, r, trunc, grads):
     prices, jac = [], []
     for T, K, _, _ in chains:
-        out = O.swift_price(list(p), S, r, T, K, trunc=trunc, grads=grads)
+        try:
+            out = O.swift_price(list(p), S, r, T, K, trunc=trunc, grads=grads)
+        except ValueError as exc:
+            raise Unpriceable(str(exc)) from exc
+        if out[1][7] > SWIFT_MAX_J:
+            raise Unpriceable(f"SWIFT grid J={int(out[1][7])} exceeds {SWIFT_MAX_J}")
         prices.append(out[0])
         if grads:
             jac.append(out[2])
@@ -134,8 +147,13 @@ def levenberg_marquardt(S, chains, p0, r=YEARLY_RISK_FREE, trunc=1e-10, itmax=10
                 if dpL2 >= (pL2 + eps2) / (EPS * EPS):
                     stop = 4
                     break
+                hn = None
                 if valid(pn):
-                    hn, _ = model(pn, S, chains, r, trunc, False)
+                    try:
+                        hn, _ = model(pn, S, chains, r, trunc, False)
+                    except Unpriceable:
+                        hn = None
+                if hn is not None:
                     nfev += 1
                     en = x - hn
                     enL2 = float(en @ en)
