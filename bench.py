@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -198,6 +199,50 @@ def calibration_leg(ctx):
                                        f"{cpu_s:.2f} s"}}
 
 
+def martingale_leg(ctx, args, plain_value: float, plain_kernel_ms: float):
+    """Andersen's martingale correction on the grid workload: the corrected FP32 kernel's
+    throughput against the plain one, FP32 vs FP64 with the correction on identical draws, and
+    the terminal-mean error it removes on a coarse stiff request."""
+    import hestonexotics_amd as H
+    from hestonexotics_amd import workloads as W
+
+    def corrected(prec):
+        reqs = W.c2(n_paths=args.paths, precision=prec)
+        for q in reqs:
+            q.flags |= H.MARTINGALE
+        return reqs
+
+    reqs = corrected(H.FP32)
+    nominal = W.path_steps(reqs)
+    b = ctx.batch(reqs)
+    wall, kms = time_batch(ctx, b, max(2, args.steps // 3), 1)
+    p32, _ = b.results()
+    b.close()
+    steps64 = max(2, args.steps // 10)
+    b64 = ctx.batch(corrected(H.FP64))
+    wall64, _ = time_batch(ctx, b64, steps64, 1)
+    p64, _ = b64.results()
+    b64.close()
+    big = p64 > 1e-2
+    rel = np.abs(p32[big] - p64[big]) / p64[big]
+    value = nominal * max(2, args.steps // 3) / wall
+    # E[X_T] against the forward: v0 0.2, kappa 0.5, theta 0.04, sigma 0.9, rho -0.9, 4 one-year steps
+    fwd = 100.0 * math.exp(0.02 * 4.0)
+    z = {}
+    for mart in (False, True):
+        q = H.Request(v0=0.2, theta=0.04, rho=-0.9, kappa=0.5, sigma=0.9, S0=100.0, r=0.02, expiries=[4.0],
+                      strikes=[[0.0]], steps=4, n_paths=1 << 24, seed=21, payoffs=H.EUROPEAN,
+                      flags=H.DRIFT | (H.MARTINGALE if mart else 0), precision=H.FP32)
+        pr, se = ctx.price(q)
+        z["corrected" if mart else "plain"] = {"rel_err": float((pr[0] - fwd) / fwd),
+                                                "err_in_se": float((pr[0] - fwd) / se[0])}
+    return {"workload": "C2 grid with HEXO_MARTINGALE (FP32 corrected-step kernel)", "value": value,
+            "kernel_ms": kms, "over_plain": value / plain_value, "kernel_ms_plain": plain_kernel_ms,
+            "fp32_vs_fp64_max_rel_err": float(rel.max()), "fp64_value": nominal * steps64 / wall64,
+            "terminal_mean_vs_forward": z,
+            "terminal_mean_request": "v0 0.2, kappa 0.5, theta 0.04, sigma 0.9, rho -0.9, T 4, 4 steps, 2^24 paths"}
+
+
 def time_batch(ctx, batch, steps: int, warmup: int):
     for _ in range(warmup):
         batch.run_eager()
@@ -234,6 +279,7 @@ def main():
     ap.add_argument("--c3-fp64-scale", type=float, default=0.05)
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c5-paths", type=int, default=2 ** 27, help="paths of the stiff precision-study request")
+    ap.add_argument("--no-martingale", action="store_true")
     ap.add_argument("--no-swift", action="store_true")
     ap.add_argument("--swift-params", type=int, default=64, help="parameter sets of the SWIFT calibration batch")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
@@ -367,6 +413,10 @@ def main():
                          "max_abs_err_in_se": float(se_units.max()), "n_prices": int(big.sum())},
         "graph_replay_ms_per_step": graph_s * 1e3,
     }
+
+    if not args.no_martingale:
+        out["martingale"] = martingale_leg(ctx, args, value, kms32)
+        log("martingale:", out["martingale"])
 
     if not args.no_c3:
         c3 = W.c3(precision=H.FP32)

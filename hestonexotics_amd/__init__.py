@@ -2,8 +2,9 @@
 
 The product is libhexo.so (hestonexotics_amd/csrc); this package is its Python binding.
 """
-from .api import (ASIAN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64, TRAPEZOID, Batch, Context,
-                  HexoError, Request, SwiftCall, SwiftChain, swift_grid, time_grid, validate)
+from .api import (ASIAN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64, MARTINGALE, TRAPEZOID, Batch,
+                  Context, HexoError, Request, SwiftCall, SwiftChain, swift_grid, time_grid, validate)
 
-__all__ = ["ASIAN", "DISCOUNT", "DRIFT", "EUROPEAN", "FP32", "FP32_COMPENSATED", "FP64", "TRAPEZOID", "Batch",
-           "Context", "HexoError", "Request", "SwiftCall", "SwiftChain", "swift_grid", "time_grid", "validate"]
+__all__ = ["ASIAN", "DISCOUNT", "DRIFT", "EUROPEAN", "FP32", "FP32_COMPENSATED", "FP64", "MARTINGALE", "TRAPEZOID",
+           "Batch", "Context", "HexoError", "Request", "SwiftCall", "SwiftChain", "swift_grid", "time_grid",
+           "validate"]

@@ -17,7 +17,7 @@ ERRORS = {
     -9: "HEXO_ERR_STATE", -10: "HEXO_ERR_INJECTED",
 }
 ASIAN, EUROPEAN = 1, 2
-DRIFT, DISCOUNT, TRAPEZOID = 1, 2, 4
+DRIFT, DISCOUNT, TRAPEZOID, MARTINGALE = 1, 2, 4, 8
 FP64, FP32, FP32_COMPENSATED = 0, 1, 2
 
 

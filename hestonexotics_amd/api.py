@@ -17,12 +17,13 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ASIAN, CALIB_INITIAL_GIVEN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64,
-                   SWIFT_FIXED_GRID, TRAPEZOID, HexoError, hexo_batch_stats, hexo_calib_chain, hexo_calib_config,
-                   hexo_calib_result, hexo_request, hexo_server_config, hexo_server_stats, hexo_swift_grid,
-                   hexo_swift_request, hexo_ticket_info, hexo_tuning)
+                   MARTINGALE, SWIFT_FIXED_GRID, TRAPEZOID, HexoError, hexo_batch_stats, hexo_calib_chain,
+                   hexo_calib_config, hexo_calib_result, hexo_request, hexo_server_config, hexo_server_stats,
+                   hexo_swift_grid, hexo_swift_request, hexo_ticket_info, hexo_tuning)
 
-__all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "DRIFT", "DISCOUNT", "TRAPEZOID", "FP64", "FP32",
-           "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "SwiftChain", "SwiftCall", "swift_grid"]
+__all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "DRIFT", "DISCOUNT", "TRAPEZOID", "MARTINGALE",
+           "FP64", "FP32", "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "SwiftChain", "SwiftCall",
+           "swift_grid"]
 
 GRID_FIELDS = ("m", "exp2_m", "sqrt_exp2_m", "lower", "upper", "k1", "k2", "J")
 

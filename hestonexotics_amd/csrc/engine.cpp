@@ -111,7 +111,7 @@ std::string validate(const hexo_request* r) {
     for (uint64_t i = 0; i < n_opts; ++i)
         if (!fin(r->strikes[i])) return "strikes must be finite";
     if (r->payoffs == 0 || (r->payoffs & ~3u)) return "payoffs must be a non-empty subset of ASIAN|EUROPEAN";
-    if (r->flags & ~7u) return "unknown flag bits";
+    if (r->flags & ~15u) return "unknown flag bits";
     if (r->precision > 2u) return "precision must be HEXO_FP64, HEXO_FP32 or HEXO_FP32_COMPENSATED";
     const uint64_t n_tiles = (r->n_paths + kTile - 1) / kTile;
     if (n_tiles * n_opts * n_payoffs(r->payoffs) > (1ull << 33)) return "n_paths * outputs too large for one request";
@@ -203,13 +203,13 @@ struct ReqPlan {
 };
 
 struct Layout {
-    size_t reqs, segs, fins, opts, order[2], item_end[2], group_end, out_end;  // uploaded
+    size_t reqs, segs, fins, opts, order[kPathClasses], item_end[kPathClasses], group_end, out_end;  // uploaded
     size_t upload_end;
     size_t counters, counts, partials, groups, prices, ses, finals;  // device only
     size_t total;
 };
 
-Layout make_layout(uint32_t n_req, uint64_t n_seg, uint64_t n_fin, uint64_t n_opt, uint32_t n_cls0, uint32_t n_cls1,
+Layout make_layout(uint32_t n_req, uint64_t n_seg, uint64_t n_fin, uint64_t n_opt, const uint32_t* n_cls,
                    uint64_t n_partials, uint64_t n_groups, uint64_t n_out, uint64_t n_finals) {
     Layout L{};
     size_t o = 0;
@@ -222,10 +222,10 @@ Layout make_layout(uint32_t n_req, uint64_t n_seg, uint64_t n_fin, uint64_t n_op
     L.segs = take(sizeof(DevSeg) * std::max<uint64_t>(n_seg, 1));
     L.fins = take(sizeof(DevFin) * std::max<uint64_t>(n_fin, 1));
     L.opts = take(sizeof(DevOpt) * std::max<uint64_t>(n_opt, 1));
-    L.order[0] = take(sizeof(uint32_t) * std::max<uint32_t>(n_cls0, 1));
-    L.item_end[0] = take(sizeof(uint32_t) * std::max<uint32_t>(n_cls0, 1));
-    L.order[1] = take(sizeof(uint32_t) * std::max<uint32_t>(n_cls1, 1));
-    L.item_end[1] = take(sizeof(uint32_t) * std::max<uint32_t>(n_cls1, 1));
+    for (uint32_t c = 0; c < kPathClasses; ++c) {
+        L.order[c] = take(sizeof(uint32_t) * std::max<uint32_t>(n_cls[c], 1));
+        L.item_end[c] = take(sizeof(uint32_t) * std::max<uint32_t>(n_cls[c], 1));
+    }
     L.group_end = take(sizeof(uint64_t) * n_req);
     L.out_end = take(sizeof(uint64_t) * n_req);
     L.upload_end = o;
@@ -247,7 +247,7 @@ bool plan_request(const hexo_request* r, ReqPlan& p) {
     p.n_out = p.n_opts * p.n_pay;
     p.n_tiles = (uint32_t)((r->n_paths + kTile - 1) / kTile);
     p.n_groups = (p.n_tiles + kGroupTiles - 1) / kGroupTiles;
-    p.cls = r->precision == HEXO_FP64 ? 0u : 1u;
+    p.cls = r->precision == HEXO_FP64 ? 0u : (r->flags & HEXO_MARTINGALE) ? 2u : 1u;
     p.n_fins = 0;
     for (auto& s : p.segs) p.n_fins += (uint32_t)s.fins.size();
     return true;
@@ -263,7 +263,7 @@ size_t request_device_bytes(const hexo_request* r) {
 }
 
 size_t batch_device_bytes(const hexo_request* reqs, uint32_t n) {
-    size_t s = 16 * 256;  // alignment slack of the region table
+    size_t s = 24 * 256;  // alignment slack of the region table
     for (uint32_t i = 0; i < n; ++i) s += request_device_bytes(&reqs[i]);
     return s;
 }
@@ -294,7 +294,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     }
     std::vector<ReqPlan> plans(n);
     uint64_t n_seg = 0, n_fin = 0, n_opt = 0, n_partials = 0, n_groups = 0, n_out = 0;
-    uint32_t n_cls[2] = {0, 0};
+    uint32_t n_cls[kPathClasses] = {};
     for (uint32_t i = 0; i < n; ++i) {
         std::string why = validate(&reqs[i]);
         if (!why.empty()) {
@@ -320,9 +320,9 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     b->n_req = n;
     b->n_out_total = n_out;
     b->ppt[0] = 1;
-    b->ppt[1] = ctx->tuning.paths_per_thread;
+    b->ppt[1] = b->ppt[2] = ctx->tuning.paths_per_thread;
     const uint64_t n_finals = finals_n ? *finals_n : 0;
-    const Layout L = make_layout(n, n_seg, n_fin, n_opt, n_cls[0], n_cls[1], n_partials, n_groups, n_out, n_finals);
+    const Layout L = make_layout(n, n_seg, n_fin, n_opt, n_cls, n_partials, n_groups, n_out, n_finals);
 
     // host staging: the uploaded tables, then the result buffers
     b->upload_bytes = L.upload_end;
@@ -415,6 +415,11 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
             g.kc = (float)(kc * L2E);
             g.kd = (float)(kd * L2E);
             g.k3 = (float)(K3 * L2E * L2E);
+            const double A = kc + kd + 0.5 * K3;  // K2 + K4/2
+            g.mA = (float)A;
+            g.ma = (float)(A * L2E);
+            g.mh = (float)(0.5 * K3 * L2E);
+            g.mr = (float)(d.r_drift * dl * L2E);
             g.delta_f = (float)dl;
             g.dnext_f = (float)S.dnext;
             g.n_steps = S.n_steps;
@@ -450,7 +455,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     b->n_out_threads = out_i;
 
     // work queues: per kernel class, requests longest first
-    for (uint32_t cls = 0; cls < 2; ++cls) {
+    for (uint32_t cls = 0; cls < kPathClasses; ++cls) {
         std::vector<uint32_t> idx;
         for (uint32_t i = 0; i < n; ++i)
             if (plans[i].cls == cls) idx.push_back(i);
@@ -497,7 +502,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     char* D = b->d_mem;
     b->d_counters = reinterpret_cast<uint32_t*>(D + L.counters);
     b->d_counts = reinterpret_cast<unsigned long long*>(D + L.counts);
-    for (uint32_t cls = 0; cls < 2; ++cls) {
+    for (uint32_t cls = 0; cls < kPathClasses; ++cls) {
         PathArgs& a = b->pa[cls];
         a.reqs = reinterpret_cast<const DevReq*>(D + L.reqs);
         a.segs = reinterpret_cast<const DevSeg*>(D + L.segs);
@@ -524,17 +529,19 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
 
     // launch shapes
     const int n_cu = ctx->prop.multiProcessorCount > 0 ? ctx->prop.multiProcessorCount : 256;
-    for (uint32_t cls = 0; cls < 2; ++cls) {
+    for (uint32_t cls = 0; cls < kPathClasses; ++cls) {
         if (!b->has_class[cls]) continue;
-        const uint32_t prec = cls == 0 ? HEXO_FP64 : HEXO_FP32;
+        const uint32_t prec = class_precision(cls);
         const uint32_t block = cls == 0 ? 256u : ctx->tuning.block_threads;
         const size_t lds = path_lds_bytes(prec, b->ppt[cls], block);
         int bpc = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, path_kernel_symbol(prec, b->ppt[cls], PathMode::Normal),
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc,
+                                                          path_kernel_symbol(prec, b->ppt[cls], PathMode::Normal,
+                                                                             class_martingale(cls)),
                                                           (int)block, lds) != hipSuccess ||
             bpc <= 0)
             bpc = std::max(1, 1024 / (int)block);
-        if (cls == 1 && ctx->tuning.waves_per_simd)
+        if (cls != 0 && ctx->tuning.waves_per_simd)
             bpc = std::max(1, std::min(bpc, (int)(ctx->tuning.waves_per_simd * 4 * 64 / block)));
         const uint64_t waves_per_block = block / kTile;
         const uint64_t need = (b->pa[cls].total_items + waves_per_block - 1) / waves_per_block;
@@ -558,15 +565,16 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
 }
 
 static int enqueue_body(Batch& b, PathMode mode, bool timed, double* d_finals, std::string& err) {
-    HIP_TRY(launch_fill_u32(b.d_counters, 2, 0u, b.stream));
+    HIP_TRY(launch_fill_u32(b.d_counters, kPathClasses, 0u, b.stream));
     if (mode == PathMode::Count) HIP_TRY(launch_zero_u64(b.d_counts, 4, b.stream));
     if (timed) HIP_TRY(hipEventRecord(b.ev[0], b.stream));
-    for (uint32_t cls = 0; cls < 2; ++cls) {
+    for (uint32_t cls = 0; cls < kPathClasses; ++cls) {
         if (!b.has_class[cls]) continue;
         PathArgs a = b.pa[cls];
         a.finals = d_finals;
         a.counts = mode == PathMode::Count ? b.d_counts : nullptr;
-        HIP_TRY(launch_paths(cls == 0 ? HEXO_FP64 : HEXO_FP32, b.ppt[cls], mode, b.grid[cls], b.block[cls], b.stream, a));
+        HIP_TRY(launch_paths(class_precision(cls), b.ppt[cls], mode, b.grid[cls], b.block[cls], b.stream, a,
+                             class_martingale(cls)));
         ++b.path_launches;
     }
     if (timed) HIP_TRY(hipEventRecord(b.ev[1], b.stream));
@@ -614,7 +622,7 @@ int batch_launch_graph(Batch& b, std::string& err) {
     b.timed = false;
     b.t_run = std::chrono::steady_clock::now();
     HIP_TRY(hipGraphLaunch(b.exec, b.stream));
-    b.path_launches += (b.has_class[0] ? 1 : 0) + (b.has_class[1] ? 1 : 0);
+    for (uint32_t cls = 0; cls < kPathClasses; ++cls) b.path_launches += b.has_class[cls] ? 1 : 0;
     return HEXO_OK;
 }
 

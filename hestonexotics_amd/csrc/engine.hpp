@@ -70,6 +70,10 @@ struct PinnedBuf {
 };
 
 // ---- a batch of requests resident on the device --------------------------------------------
+constexpr uint32_t kPathClasses = 3;
+inline uint32_t class_precision(uint32_t cls) { return cls == 0 ? HEXO_FP64 : HEXO_FP32; }
+inline bool class_martingale(uint32_t cls) { return cls == 2; }
+
 struct Batch {
     hexo_ctx* ctx = nullptr;
     hipStream_t stream = nullptr;
@@ -88,10 +92,11 @@ struct Batch {
     size_t upload_bytes = 0;
     double* h_prices = nullptr;
     double* h_ses = nullptr;
-    // kernels: class 0 = FP64 kernel, class 1 = FP32 kernel
-    PathArgs pa[2]{};
-    uint32_t grid[2]{}, block[2]{}, ppt[2]{};
-    bool has_class[2]{};
+    // kernel classes: 0 = FP64 kernel, 1 = FP32 kernel, 2 = FP32 kernel with the
+    // martingale-corrected step
+    PathArgs pa[kPathClasses]{};
+    uint32_t grid[kPathClasses]{}, block[kPathClasses]{}, ppt[kPathClasses]{};
+    bool has_class[kPathClasses]{};
     ReduceArgs ra{};
     uint64_t n_group_threads = 0, n_out_threads = 0;
     uint32_t* d_counters = nullptr;

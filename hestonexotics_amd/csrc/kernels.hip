@@ -26,6 +26,7 @@ namespace {
 
 constexpr uint32_t kAsian = 1u;
 constexpr uint32_t kTrapezoid = 4u;
+constexpr uint32_t kMartingale = 8u;
 constexpr uint32_t kPrecFP64 = 0u;
 constexpr uint32_t kPrecComp = 2u;
 
@@ -224,6 +225,7 @@ __device__ __forceinline__ void count_flush(unsigned long long* counts, uint32_t
 // ------------------------------------------------------------------------------------------
 struct SegF {
     float omd, th, c1, c2, g0, kc, kd, k3;
+    float mA, ma, mh, mr;
 };
 
 // Per-wave LDS scratch of the FP32 kernel: the tail-normal queue during the steps (2*PPT words in,
@@ -274,7 +276,9 @@ struct StepCount {
 // variance by the quadratic branch a (b + Z_V)^2 when psi = s2/m^2 < 1.5, else the exponential
 // branch; then the central log-price step. Y is log2(X/S0); X = 2^Y; P accumulates X for the
 // Asian sum. Z_V (quadratic lanes) and Z_X come from AS241 with the tails gathered per wave.
-template <int PPT, bool COUNT>
+// MART: Andersen's martingale-corrected step (the DevSeg form), falling back per path and step to
+// the plain step where E[exp(A V')] does not exist.
+template <int PPT, bool COUNT, bool MART>
 __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], float (&X)[PPT], float (&Xp)[PPT],
                                              float (&P)[PPT], const uint32_t (&wv)[PPT], const uint32_t (&wx)[PPT],
                                              const SegF& c, uint32_t* q, int lane, const bool (&valid)[PPT],
@@ -314,6 +318,10 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         float Vn;
+        // MART: A m - ln E[exp(A V')] = fs f(ft) + corr with f(t) = t - ln(1 + t), in natural
+        // log units; mok: the expectation exists
+        float ft = 0.0f, fs = 0.0f, corr = 0.0f;
+        bool mok = false;
         if (quad[j]) {
             const float inv = __builtin_amdgcn_rcpf(s2[j]);
             const float tt = (m2[j] + m2[j]) * inv;  // 2 / psi
@@ -326,6 +334,14 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
             const float aq = m[j] * rcp_refined(__builtin_fmaf(b, b, 1.0f));
             const float bz = b + z[2 * j];
             Vn = aq * bz * bz;
+            if constexpr (MART) {
+                // x = 2 A a < 1: A m - ln E[exp(A V')] = -(f(-x) + x^2 b^2 / (1 - x)) / 2
+                const float x = (c.mA + c.mA) * aq;
+                mok = x < 1.0f;
+                ft = -x;
+                fs = -0.5f;
+                corr = -0.5f * (x * x) * b2 * __builtin_amdgcn_rcpf(1.0f - x);
+            }
         } else {
             // p = (psi-1)/(psi+1), beta = 2/(m (psi+1)); V' = ln((1-p)/(1-U))/beta if U > p, else 0.
             // With sm = s2 + m^2: 1-p = 2m^2/sm and 1/beta = sm/(2m).
@@ -333,12 +349,39 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
             const float lhs = one_minus_uniform_f32(wv[j]) * sm;  // (1-U) sm
             const float tm2 = m2[j] + m2[j];
             const float lg = __builtin_amdgcn_logf(tm2 * rcp_refined(lhs));
-            const float v = (sm * rcp_refined(m[j])) * (0.5f * 0.693147180559945309f) * lg;
+            const float ib = sm * rcp_refined(m[j]);  // 2 / beta
+            const float v = ib * (0.5f * 0.693147180559945309f) * lg;
             Vn = lhs < tm2 ? v : 0.0f;
+            if constexpr (MART) {
+                // cA = A / beta < 1, r1 = (1-p) / (1 - cA) = 2 m^2 / (sm (1 - cA)), y = r1 cA:
+                // A m - ln E[exp(A V')] = f(y) - r1 cA^2
+                const float cA = (0.5f * c.mA) * ib;
+                mok = cA < 1.0f;
+                const float r1 = tm2 * __builtin_amdgcn_rcpf(__builtin_fmaf(-cA, sm, sm));
+                ft = r1 * cA;
+                fs = 1.0f;
+                corr = -ft * cA;
+            }
         }
         const float sum = V[j] + Vn;
         const float dif = Vn - V[j];
-        const float inc = __builtin_fmaf(c.kd, dif, __builtin_fmaf(c.kc, sum - (c.th + c.th), c.g0));
+        float inc = __builtin_fmaf(c.kd, dif, __builtin_fmaf(c.kc, sum - (c.th + c.th), c.g0));
+        if constexpr (MART) {
+            // f(t) = t - ln(1 + t): its series through t^7 for |t| < 1/16 (relative error < 2e-8),
+            // else from the rounded 1 + t. The direct form alone would return f = t wherever 1 + t
+            // rounds to 1, a one-signed error of up to 6e-8 per step where t stays small.
+            const float t2 = ft * ft;
+            float h = __builtin_fmaf(ft, -1.0f / 7.0f, 1.0f / 6.0f);
+            h = __builtin_fmaf(ft, h, -0.2f);
+            h = __builtin_fmaf(ft, h, 0.25f);
+            h = __builtin_fmaf(ft, h, -1.0f / 3.0f);
+            h = __builtin_fmaf(ft, h, 0.5f);
+            const float fdir = __builtin_fmaf(-0.693147180559945309f, __builtin_amdgcn_logf(1.0f + ft), ft);
+            const float f = __builtin_fabsf(ft) < 0.0625f ? t2 * h : fdir;
+            const float cl = __builtin_fmaf(fs, f, corr) * 1.44269504088896341f;
+            const float im = __builtin_fmaf(c.ma, Vn - m[j], __builtin_fmaf(-c.mh, sum, c.mr)) + cl;
+            inc = mok ? im : inc;
+        }
         const float sq = __builtin_amdgcn_sqrtf(c.k3 * sum);
         Y[j] = __builtin_fmaf(sq, z[2 * j + 1], Y[j] + inc);
         Xp[j] = X[j];
@@ -362,7 +405,7 @@ __device__ __forceinline__ void philox_block(uint32_t blk, const uint32_t (&plo)
     }
 }
 
-template <int PPT, bool DEBUG, bool COUNT>
+template <int PPT, bool DEBUG, bool COUNT, bool MART>
 __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32_t tile0, int lane, uint32_t* lds) {
     const DevReq& R = a.reqs[ri];
     const uint64_t n_paths = R.n_paths;
@@ -393,24 +436,24 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
     const uint32_t n_segs = R.n_segs;
     for (uint32_t s = 0; s < n_segs; ++s) {
         const DevSeg& S = a.segs[R.seg_begin + s];
-        const SegF c{S.omd, S.th, S.c1, S.c2, S.g0, S.kc, S.kd, S.k3};
+        const SegF c{S.omd, S.th, S.c1, S.c2, S.g0, S.kc, S.kd, S.k3, S.mA, S.ma, S.mh, S.mr};
         const uint32_t n = S.n_steps;
         uint32_t i = 0;
         if ((k & 1u) && n > 0) {  // second half of a block opened by the previous run
-            qe_step_fp32<PPT, COUNT>(V, Y, X, Xp, P, sv, sx, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART>(V, Y, X, Xp, P, sv, sx, c, lds, lane, valid, cnt);
             ++i;
             ++k;
         }
         for (; i + 2 <= n; i += 2, k += 2) {
             uint32_t w0[PPT], w1[PPT], w2[PPT], w3[PPT];
             philox_block<PPT>(k >> 1, plo, phi, key0, key1, w0, w1, w2, w3);
-            qe_step_fp32<PPT, COUNT>(V, Y, X, Xp, P, w0, w1, c, lds, lane, valid, cnt);
-            qe_step_fp32<PPT, COUNT>(V, Y, X, Xp, P, w2, w3, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART>(V, Y, X, Xp, P, w0, w1, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART>(V, Y, X, Xp, P, w2, w3, c, lds, lane, valid, cnt);
         }
         if (i < n) {
             uint32_t w0[PPT], w1[PPT];
             philox_block<PPT>(k >> 1, plo, phi, key0, key1, w0, w1, sv, sx);
-            qe_step_fp32<PPT, COUNT>(V, Y, X, Xp, P, w0, w1, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART>(V, Y, X, Xp, P, w0, w1, c, lds, lane, valid, cnt);
             ++k;
         }
         // Asian accumulation of the run's interior steps b..e-1:
@@ -454,7 +497,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
     if constexpr (COUNT) count_flush(a.counts, cnt.quad, cnt.exp, cnt.central, cnt.tail, lane);
 }
 
-template <int PPT, bool DEBUG, bool COUNT>
+template <int PPT, bool DEBUG, bool COUNT, bool MART>
 __global__ __launch_bounds__(256) void k_path_fp32(PathArgs a) {
     extern __shared__ uint32_t lds_u32[];
     const int lane = (int)(threadIdx.x & 63u);
@@ -465,7 +508,7 @@ __global__ __launch_bounds__(256) void k_path_fp32(PathArgs a) {
         const uint32_t sp = uniform(find_sorted(a.item_end, a.n_sorted, item));
         const uint32_t first = sp ? a.item_end[sp - 1] : 0u;
         const uint32_t ri = uniform(a.order[sp]);
-        path_fp32<PPT, DEBUG, COUNT>(a, ri, uniform((item - first) * (uint32_t)PPT), lane, lds);
+        path_fp32<PPT, DEBUG, COUNT, MART>(a, ri, uniform((item - first) * (uint32_t)PPT), lane, lds);
     }
 }
 // ------------------------------------------------------------------------------------------
@@ -482,23 +525,35 @@ __device__ __forceinline__ bool qe_step_fp64(const DevReq& R, double delta, doub
     const double psi = s2 / (m * m);
     const double Vp = V;
     const bool quad = psi < 1.5;
+    double aq = 0.0, bp2 = 0.0, p = 0.0, beta = 0.0;
     if (quad) {
-        const double bp2 = 2.0 / psi - 1.0 + ::sqrt(2.0 / psi * (2.0 / psi - 1.0));
+        bp2 = 2.0 / psi - 1.0 + ::sqrt(2.0 / psi * (2.0 / psi - 1.0));
         const double b = ::sqrt(bp2);
-        const double aq = m / (1.0 + bp2);
+        aq = m / (1.0 + bp2);
         const double zv = normal_f64(wv);
         V = aq * (b + zv) * (b + zv);
     } else {
-        const double p = (psi - 1.0) / (psi + 1.0);
-        const double beta = 2.0 / (m * (psi + 1.0));
+        p = (psi - 1.0) / (psi + 1.0);
+        beta = 2.0 / (m * (psi + 1.0));
         const double u = uniform_f64(wv);
         V = p < u ? ::log((1.0 - p) / (1.0 - u)) / beta : 0.0;
     }
-    const double k0 = -rho * kappa * theta / eps * delta + R.r_drift * delta;
+    double k0 = -rho * kappa * theta / eps * delta + R.r_drift * delta;
     const double k1 = 0.5 * delta * (kappa * rho / eps - 0.5) - rho / eps;
     const double k2 = 0.5 * delta * (kappa * rho / eps - 0.5) + rho / eps;
     const double k3 = 0.5 * delta * (1.0 - rho * rho);
     const double k4 = 0.5 * delta * (1.0 - rho * rho);
+    if (R.flags & kMartingale) {
+        // Andersen's martingale-corrected K0* = -ln E[exp(A V') | V] - (K1 + K3/2) V, A = K2 + K4/2
+        const double A = k2 + 0.5 * k4;
+        if (quad && 2.0 * A * aq < 1.0) {
+            const double lnM = A * bp2 * aq / (1.0 - 2.0 * A * aq) - 0.5 * ::log1p(-2.0 * A * aq);
+            k0 = -lnM - (k1 + 0.5 * k3) * Vp + R.r_drift * delta;
+        } else if (!quad && A < beta) {
+            const double lnM = ::log(p + beta * (1.0 - p) / (beta - A));
+            k0 = -lnM - (k1 + 0.5 * k3) * Vp + R.r_drift * delta;
+        }
+    }
     logX = logX + k0 + k1 * Vp + k2 * V + ::sqrt(k3 * Vp + k4 * V) * normal_f64(wx);
     return quad;
 }
@@ -741,7 +796,7 @@ inline dim3 grid_for(uint64_t n, uint32_t block) { return dim3((uint32_t)((n + b
 // ------------------------------------------------------------------------------------------
 // Launch wrappers (host)
 // ------------------------------------------------------------------------------------------
-const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode) {
+const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode, bool mart) {
     if (precision == kPrecFP64) {
         switch (mode) {
             case PathMode::Normal: return reinterpret_cast<const void*>(&k_path_fp64<false, false>);
@@ -750,14 +805,19 @@ const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode) 
         }
         return nullptr;
     }
-#define HEXO_PPT_CASE(P)                                                                           \
-    case P:                                                                                        \
-        switch (mode) {                                                                            \
-            case PathMode::Normal: return reinterpret_cast<const void*>(&k_path_fp32<P, false, false>); \
-            case PathMode::Debug: return reinterpret_cast<const void*>(&k_path_fp32<P, true, false>);   \
-            case PathMode::Count: return reinterpret_cast<const void*>(&k_path_fp32<P, false, true>);   \
-        }                                                                                          \
-        return nullptr;
+#define HEXO_PPT_MODES(P, M)                                                                         \
+    switch (mode) {                                                                                  \
+        case PathMode::Normal: return reinterpret_cast<const void*>(&k_path_fp32<P, false, false, M>); \
+        case PathMode::Debug: return reinterpret_cast<const void*>(&k_path_fp32<P, true, false, M>);   \
+        case PathMode::Count: return reinterpret_cast<const void*>(&k_path_fp32<P, false, true, M>);   \
+    }                                                                                                \
+    return nullptr;
+#define HEXO_PPT_CASE(P)          \
+    case P:                       \
+        if (mart) {               \
+            HEXO_PPT_MODES(P, true)  \
+        }                         \
+        HEXO_PPT_MODES(P, false)
     switch (ppt) {
         HEXO_PPT_CASE(1)
         HEXO_PPT_CASE(2)
@@ -765,6 +825,7 @@ const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode) 
         default: return nullptr;
     }
 #undef HEXO_PPT_CASE
+#undef HEXO_PPT_MODES
 }
 
 size_t path_lds_bytes(uint32_t precision, uint32_t ppt, uint32_t block) {
@@ -774,7 +835,7 @@ size_t path_lds_bytes(uint32_t precision, uint32_t ppt, uint32_t block) {
 }
 
 hipError_t launch_paths(uint32_t precision, uint32_t ppt, PathMode mode, uint32_t grid, uint32_t block,
-                        hipStream_t st, const PathArgs& a) {
+                        hipStream_t st, const PathArgs& a, bool mart) {
     const size_t lds = path_lds_bytes(precision, ppt, block);
     if (precision == kPrecFP64) {
         switch (mode) {
@@ -784,11 +845,17 @@ hipError_t launch_paths(uint32_t precision, uint32_t ppt, PathMode mode, uint32_
         }
         return hipGetLastError();
     }
-#define HEXO_LAUNCH(P)                                                                        \
-    switch (mode) {                                                                           \
-        case PathMode::Normal: k_path_fp32<P, false, false><<<grid, block, lds, st>>>(a); break; \
-        case PathMode::Debug: k_path_fp32<P, true, false><<<grid, block, lds, st>>>(a); break;   \
-        case PathMode::Count: k_path_fp32<P, false, true><<<grid, block, lds, st>>>(a); break;   \
+#define HEXO_LAUNCH_M(P, M)                                                                      \
+    switch (mode) {                                                                              \
+        case PathMode::Normal: k_path_fp32<P, false, false, M><<<grid, block, lds, st>>>(a); break; \
+        case PathMode::Debug: k_path_fp32<P, true, false, M><<<grid, block, lds, st>>>(a); break;   \
+        case PathMode::Count: k_path_fp32<P, false, true, M><<<grid, block, lds, st>>>(a); break;   \
+    }
+#define HEXO_LAUNCH(P)            \
+    if (mart) {                   \
+        HEXO_LAUNCH_M(P, true)    \
+    } else {                      \
+        HEXO_LAUNCH_M(P, false)   \
     }
     switch (ppt) {
         case 1: HEXO_LAUNCH(1) break;
@@ -797,6 +864,7 @@ hipError_t launch_paths(uint32_t precision, uint32_t ppt, PathMode mode, uint32_
         default: return hipErrorInvalidValue;
     }
 #undef HEXO_LAUNCH
+#undef HEXO_LAUNCH_M
     return hipGetLastError();
 }
 

@@ -29,9 +29,16 @@ constexpr int kMaxPPT = 4;        // paths per thread (FP32 kernel)
 // the constants to FP32 moves neither fixed point: in the form V e^-kappa dt + theta(1 - e^-kappa dt)
 // the rounded e^-kappa dt shifts the mean-reversion level by O(ulp / (kappa dt)), and K0 and
 // K1 V + K2 V' nearly cancel, so their separate roundings add a drift of O(ulp K0) per step.
+//
+// With the martingale correction the log-price step is written without K0 and K1:
+//   y += ma (V' - m) - mh (V + V') + (A m - ln E[exp(A V')]) log2(e) + mr + sqrt(k3 (V + V')) Z
+// with A = K2 + K4/2 (mA), ma = A log2(e), mh = K3/2 log2(e), mr = r dt log2(e). E[V'] = m holds
+// by construction, so the first term has mean zero whatever the rounding of ma, and
+// A m - ln E[exp(A V')] is O((A a)^2), evaluated in forms without cancellation.
 struct DevSeg {
     float omd, th, c1, c2;
     float g0, kc, kd, k3;
+    float mA, ma, mh, mr;  // martingale-corrected step (HEXO_MARTINGALE)
     float delta_f;  // Asian weight of accumulations inside the run
     float dnext_f;  // weight of the accumulation after the run's last step (new dt), 0 if none
     uint32_t n_steps;

@@ -362,7 +362,11 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
                     (!grads || hipMemcpyAsync(host + 8 * n_strk, a.grads, 40 * n_strk, hipMemcpyDeviceToHost, st) ==
                                    hipSuccess) &&
                     hipStreamSynchronize(st) == hipSuccess;
-    if (!ok) return ctx->set_error(HEXO_ERR_HIP, "SWIFT launch failed");
+    if (!ok) {
+        // drain what was queued before the pinned stage is reused
+        (void)hipStreamSynchronize(st);
+        return ctx->set_error(HEXO_ERR_HIP, "SWIFT launch failed");
+    }
     std::memcpy(prices, host, 8 * n_strk);
     if (grads) std::memcpy(grads, host + 8 * n_strk, 40 * n_strk);
     if (grids_used) std::memcpy(grids_used, grids.data(), sizeof(hexo_swift_grid) * n);

@@ -47,6 +47,12 @@ typedef enum hexo_status {
 #define HEXO_DRIFT 1u     /* add r*delta to the log-price step (the reference has no drift term) */
 #define HEXO_DISCOUNT 2u  /* multiply each price by exp(-r T) (the reference does not discount) */
 #define HEXO_TRAPEZOID 4u /* corrected trapezoid for the Asian average's last interval */
+/* Andersen's martingale correction (variance reduction, suggested beside src/inc/HSimulation.h:48-60):
+ * K0 of each step is replaced by K0* = -ln E[exp(A V') | V] - (K1 + K3/2) V with A = K2 + K4/2,
+ * so that E[X' | X, V] = X exp(r delta) exactly (the no-drift reference form: E[X'] = X). Where
+ * the expectation does not exist (A >= 1/(2a) in the quadratic branch, A >= beta in the
+ * exponential one) that step keeps the reference's K0. */
+#define HEXO_MARTINGALE 8u
 
 /* precision of the path kernel */
 #define HEXO_FP64 0u             /* plain FP64 kernel: reference form, FP64 everywhere */
@@ -76,7 +82,7 @@ typedef struct hexo_request {
     const uint32_t* n_strikes;
     const double* strikes;
     uint32_t payoffs;   /* HEXO_ASIAN | HEXO_EUROPEAN */
-    uint32_t flags;     /* HEXO_DRIFT | HEXO_DISCOUNT | HEXO_TRAPEZOID */
+    uint32_t flags;     /* HEXO_DRIFT | HEXO_DISCOUNT | HEXO_TRAPEZOID | HEXO_MARTINGALE */
     uint32_t precision; /* HEXO_FP64 | HEXO_FP32 | HEXO_FP32_COMPENSATED */
     uint32_t reserved;  /* must be 0 */
 } hexo_request;

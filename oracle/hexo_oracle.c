@@ -269,42 +269,84 @@ float orc_normal_f32_from_word(uint32_t k) {
  * quadratic-exponential variance step with switch at PSI_C = 1.5 (src/inc/HSimulation.h:12)
  * and the central (gamma1 = gamma2 = 1/2) log-price step. The V-draw uses word w_v (as a
  * normal in the quadratic branch, as a uniform in the exponential branch); Z_X uses w_x.
- * r_drift is added as r*delta (0 reproduces the reference).
+ * r_drift is added as r*delta (0 reproduces the reference). With martingale set, K0 is
+ * Andersen's martingale-corrected K0* (L. Andersen, "Simple and efficient simulation of the
+ * Heston stochastic volatility model", J. Comput. Finance 11(3), 2008; an extension the
+ * reference suggests at src/inc/HSimulation.h:48-60 but does not implement).
  * ------------------------------------------------------------------------------------------ */
-int orc_qe_step(const double params[5], double r_drift, double delta, double* Vio, double* logXio, uint32_t w_v,
-                uint32_t w_x) {
+int orc_qe_step(const double params[5], double r_drift, double delta, int martingale, double* Vio, double* logXio,
+                uint32_t w_v, uint32_t w_x) {
     const double theta = params[1], rho = params[2], kappa = params[3], eps = params[4];
     const double V = *Vio;
     const double e = exp(-kappa * delta);
     const double m = theta + (V - theta) * e;
     const double s2 = fabs(V * eps * eps * e / kappa * (1.0 - e) + theta * eps * eps / (2.0 * kappa) * (1.0 - e) * (1.0 - e));
     const double psi = s2 / (m * m);
-    double Vn;
+    double Vn, a = 0.0, b2 = 0.0, p = 0.0, beta = 0.0;
     int quad;
     if (psi < 1.5) {
         const double two_over_psi = 2.0 / psi;
-        const double b2 = two_over_psi - 1.0 + sqrt(two_over_psi * (two_over_psi - 1.0));
+        b2 = two_over_psi - 1.0 + sqrt(two_over_psi * (two_over_psi - 1.0));
         const double b = sqrt(b2);
-        const double a = m / (1.0 + b2);
+        a = m / (1.0 + b2);
         const double zv = orc_normal_f64_from_word(w_v);
         Vn = a * (b + zv) * (b + zv);
         quad = 1;
     } else {
-        const double p = (psi - 1.0) / (psi + 1.0);
-        const double beta = 2.0 / (m * (psi + 1.0));
+        p = (psi - 1.0) / (psi + 1.0);
+        beta = 2.0 / (m * (psi + 1.0));
         const double u = orc_uniform_from_word(w_v);
         Vn = (p < u) ? log((1.0 - p) / (1.0 - u)) / beta : 0.0;
         quad = 0;
     }
-    const double k0 = -rho * kappa * theta / eps * delta + r_drift * delta;
+    double k0 = -rho * kappa * theta / eps * delta + r_drift * delta;
     const double k1 = 0.5 * delta * (kappa * rho / eps - 0.5) - rho / eps;
     const double k2 = 0.5 * delta * (kappa * rho / eps - 0.5) + rho / eps;
     const double k3 = 0.5 * delta * (1.0 - rho * rho);
     const double k4 = k3;
+    if (martingale) {
+        /* Andersen's martingale correction: K0* = -ln M - (K1 + K3/2) V with M = E[exp(A V') | V],
+         * A = K2 + K4/2. Quadratic branch: M = exp(A b^2 a / (1 - 2 A a)) / sqrt(1 - 2 A a) for
+         * 2 A a < 1; exponential branch: M = p + beta (1 - p) / (beta - A) for A < beta. */
+        const double A = k2 + 0.5 * k4;
+        double lnM = 0.0;
+        int ok = 0;
+        if (quad && 2.0 * A * a < 1.0) {
+            lnM = A * b2 * a / (1.0 - 2.0 * A * a) - 0.5 * log1p(-2.0 * A * a);
+            ok = 1;
+        } else if (!quad && A < beta) {
+            lnM = log(p + beta * (1.0 - p) / (beta - A));
+            ok = 1;
+        }
+        if (ok) k0 = -lnM - (k1 + 0.5 * k3) * V + r_drift * delta;
+    }
     const double zx = orc_normal_f64_from_word(w_x);
     *logXio = *logXio + k0 + k1 * V + k2 * Vn + sqrt(k3 * V + k4 * Vn) * zx;
     *Vio = Vn;
     return quad;
+}
+
+/* Conditional mean E[X'/X | V] of one step (r_drift included) by the product midpoint rule over
+ * the step's two uniforms: 2^log2_nv x 2^log2_nx words, each the midpoint of its interval. A
+ * test helper for the martingale correction, under which the mean is exp(r_drift delta). */
+double orc_qe_step_mean(const double params[5], double r_drift, double delta, int martingale, double V,
+                        uint32_t log2_nv, uint32_t log2_nx) {
+    if (log2_nv < 1 || log2_nv > 24 || log2_nx < 1 || log2_nx > 24) return NAN;
+    const uint32_t nv = 1u << log2_nv, nx = 1u << log2_nx, sv = 32 - log2_nv, sx = 32 - log2_nx;
+    double total = 0.0;
+#pragma omp parallel for reduction(+ : total) schedule(static)
+    for (int64_t i = 0; i < (int64_t)nv; ++i) {
+        const uint32_t wv = ((uint32_t)i << sv) | (1u << (sv - 1));
+        double row = 0.0;
+        for (uint32_t j = 0; j < nx; ++j) {
+            const uint32_t wx = (j << sx) | (1u << (sx - 1));
+            double v = V, lx = 0.0;
+            orc_qe_step(params, r_drift, delta, martingale, &v, &lx, wv, wx);
+            row += exp(lx);
+        }
+        total += row / nx;
+    }
+    return total / nv;
 }
 
 /* ------------------------------------------------------------------------------------------
@@ -381,6 +423,7 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
                           const uint32_t* n_strikes, const uint32_t* opt_off, const double* strikes, uint64_t n_opts,
                           acc2* sums, acc2* sumsq, double* finals, uint64_t* counts) {
     const double r_drift = (flags & ORC_DRIFT) ? r : 0.0;
+    const int mart = (flags & ORC_MARTINGALE) != 0;
     /* state as in SDE_state<2> + HQEAnderson::log_X (src/inc/SDE.h:9-14, src/HSimulation.tpp:87-94) */
     double X = S0, Xp = S0, V = params[0], logX = log(S0);
     double t = 0.0, tp = 0.0;
@@ -402,7 +445,7 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
             stream_block(seed, 0u, path, have, blk);                                           \
         }                                                                                      \
         uint32_t wv = blk[w & 3], wx = blk[(w + 1) & 3];                                       \
-        int qd = orc_qe_step(params, r_drift, dt, &V, &logX, wv, wx);                          \
+        int qd = orc_qe_step(params, r_drift, dt, mart, &V, &logX, wv, wx);                    \
         nq += qd;                                                                              \
         ne += 1 - qd;                                                                          \
         Xp = X;                                                                                \
@@ -553,7 +596,7 @@ void orc_qe_moments(const double params[5], double delta, double V, uint64_t n, 
             uint32_t blk[4];
             stream_block(seed, 0u, i, 0, blk);
             double v = V, lx = 0.0;
-            int q = orc_qe_step(params, 0.0, delta, &v, &lx, blk[0], blk[1]);
+            int q = orc_qe_step(params, 0.0, delta, 0, &v, &lx, blk[0], blk[1]);
             if (pass == 0) {
                 s1 += v;
                 continue;

@@ -39,6 +39,7 @@ extern "C" {
 #define ORC_DRIFT 1u     /* add r*delta to the log-price step (extension; reference has r = 0) */
 #define ORC_DISCOUNT 2u  /* multiply the estimator by exp(-r T) (extension) */
 #define ORC_TRAPEZOID 4u /* corrected trapezoid Asian average (extension) */
+#define ORC_MARTINGALE 8u /* Andersen's martingale-corrected K0 (extension) */
 
 /* ---- Philox4x32-10 ---- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
@@ -58,10 +59,14 @@ void orc_ppnd7_coefficients(double out[19]);      /* a0..a3,b1..b3,c0..c3,d1,d2,
 void orc_as241_coefficients(double out[45]);      /* a0..a7,b1..b7,c0..c7,d1..d7,e0..e7,f1..f7 */
 
 /* ---- one QE step (reference form, FP64) ----
- * params = {v0, theta, rho, kappa, sigma} (HParams order). Returns 1 for the quadratic
- * branch, 0 for the exponential branch. */
-int orc_qe_step(const double params[5], double r_drift, double delta, double* V, double* logX, uint32_t w_v,
-                uint32_t w_x);
+ * params = {v0, theta, rho, kappa, sigma} (HParams order). martingale != 0 applies Andersen's
+ * martingale-corrected K0 (ORC_MARTINGALE). Returns 1 for the quadratic branch, 0 for the
+ * exponential branch. */
+int orc_qe_step(const double params[5], double r_drift, double delta, int martingale, double* V, double* logX,
+                uint32_t w_v, uint32_t w_x);
+/* E[X'/X | V] of one step by a 2^log2_nv x 2^log2_nx midpoint rule over the two uniforms. */
+double orc_qe_step_mean(const double params[5], double r_drift, double delta, int martingale, double V,
+                        uint32_t log2_nv, uint32_t log2_nx);
 
 /* ---- the reference time grid for one path (path independent) ----
  * Replays the double-precision loop of src/HSimulation.tpp:35-44 and reports, for every

@@ -17,7 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ASIAN, EUROPEAN = 1, 2
-DRIFT, DISCOUNT, TRAPEZOID = 1, 2, 4
+DRIFT, DISCOUNT, TRAPEZOID, MARTINGALE = 1, 2, 4, 8
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -38,9 +38,11 @@ def _bind(lib: C.CDLL) -> C.CDLL:
     lib.orc_normal_f32_from_word.restype = C.c_float
     lib.orc_as241_coefficients.argtypes = [_f64p]
     lib.orc_ppnd7_coefficients.argtypes = [_f64p]
-    lib.orc_qe_step.argtypes = [_f64p, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                C.c_uint32, C.c_uint32]
+    lib.orc_qe_step.argtypes = [_f64p, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double), C.c_uint32, C.c_uint32]
     lib.orc_qe_step.restype = C.c_int
+    lib.orc_qe_step_mean.argtypes = [_f64p, C.c_double, C.c_double, C.c_int, C.c_double, C.c_uint32, C.c_uint32]
+    lib.orc_qe_step_mean.restype = C.c_double
     lib.orc_time_grid.argtypes = [_f64p, C.c_uint32, _u32p, C.c_uint32, _u64p, _f64p]
     lib.orc_time_grid.restype = C.c_int64
     lib.orc_price.argtypes = [_f64p, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64,
@@ -145,11 +147,20 @@ def as241_coefficients() -> np.ndarray:
     return out
 
 
-def qe_step(params, delta: float, V: float, logX: float, w_v: int, w_x: int, r_drift: float = 0.0):
+def qe_step(params, delta: float, V: float, logX: float, w_v: int, w_x: int, r_drift: float = 0.0,
+            martingale: bool = False):
     v = C.c_double(V)
     lx = C.c_double(logX)
-    quad = load().orc_qe_step(np.asarray(params, np.float64), r_drift, delta, C.byref(v), C.byref(lx), w_v, w_x)
+    quad = load().orc_qe_step(np.asarray(params, np.float64), r_drift, delta, int(martingale), C.byref(v),
+                              C.byref(lx), w_v, w_x)
     return v.value, lx.value, quad
+
+
+def qe_step_mean(params, delta: float, V: float, r_drift: float = 0.0, martingale: bool = False,
+                 log2_nv: int = 12, log2_nx: int = 12) -> float:
+    """E[X'/X | V] of one QE step by a midpoint rule over its two uniforms."""
+    return load().orc_qe_step_mean(np.asarray(params, np.float64), r_drift, delta, int(martingale), V, log2_nv,
+                                   log2_nx)
 
 
 def time_grid(T, n_strikes, steps: int):
