@@ -428,6 +428,8 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
     double X = S0, Xp = S0, V = params[0], logX = log(S0);
     double t = 0.0, tp = 0.0;
     double acc = 0.0;          /* AAsianCallNonAdaptive::accumulated_value */
+    double acc_y = 0.0, acc_w = 0.0; /* the same accumulation on ln(X/S0), and its weight sum */
+    const double logS0 = log(S0);
     double dt = T[0] / steps;  /* update_earliest(T_0, steps) */
     uint32_t c = 0;
     uint64_t priced = 0;
@@ -435,6 +437,8 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
     uint32_t blk[4];
     uint64_t have = ~(uint64_t)0;
     const int want_asian = (payoffs & ORC_ASIAN) != 0, want_euro = (payoffs & ORC_EUROPEAN) != 0;
+    const int want_cv = (payoffs & ORC_ASIAN_GEO_CV) != 0;
+    double logXp = logX;
     uint64_t nq = 0, ne = 0;
 
 #define ORC_STEP()                                                                             \
@@ -445,9 +449,11 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
             stream_block(seed, 0u, path, have, blk);                                           \
         }                                                                                      \
         uint32_t wv = blk[w & 3], wx = blk[(w + 1) & 3];                                       \
+        const double logXp_next = logX;                                                        \
         int qd = orc_qe_step(params, r_drift, dt, mart, &V, &logX, wv, wx);                    \
         nq += qd;                                                                              \
         ne += 1 - qd;                                                                          \
+        logXp = logXp_next;                                                                    \
         Xp = X;                                                                                \
         X = exp(logX);                                                                         \
         tp = t;                                                                                \
@@ -470,6 +476,21 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
                 finals[(size_t)c * 2 + 0] = avg;
                 finals[(size_t)c * 2 + 1] = xt;
             }
+            /* geometric counterpart: the same functional on y = ln(X/S0), normalised by its
+             * weight sum W; G = W S0 exp(L / W) */
+            double geo = 0.0;
+            if (want_cv) {
+                const double y = logX - logS0, yp = logXp - logS0, wf = (T[c] - tp) / dt;
+                double L, W;
+                if (flags & ORC_TRAPEZOID) {
+                    L = (acc_y + (T[c] - tp) * 0.5 * (yp + (yp + (y - yp) * wf))) / T[c];
+                    W = (acc_w + (T[c] - tp)) / T[c];
+                } else {
+                    L = (acc_y + (y - yp) * wf) / T[c];
+                    W = acc_w / T[c];
+                }
+                geo = W * S0 * exp(L / W);
+            }
             for (uint32_t j = 0; j < n_strikes[c]; ++j) {
                 const double K = strikes[opt_off[c] + j];
                 uint64_t o = opt_off[c] + j;
@@ -483,6 +504,12 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
                     double pay = xt - K > 0.0 ? xt - K : 0.0;
                     acc_add(&sums[o], pay);
                     acc_add(&sumsq[o], pay * pay);
+                    o += n_opts;
+                }
+                if (want_cv) {
+                    const double pa = avg - K > 0.0 ? avg - K : 0.0, pg = geo - K > 0.0 ? geo - K : 0.0;
+                    acc_add(&sums[o], pa - pg);
+                    acc_add(&sumsq[o], (pa - pg) * (pa - pg));
                 }
             }
             priced += n_strikes[c];
@@ -493,6 +520,8 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
         }
         if (priced >= n_opts) break; /* the reference's one extra accumulate+step has no effect */
         acc += dt * 0.5 * (X + Xp); /* accumulate_value */
+        acc_y += dt * 0.5 * ((logX - logS0) + (logXp - logS0));
+        acc_w += dt;
         ORC_STEP();
     }
 #undef ORC_STEP
@@ -508,7 +537,9 @@ int orc_price(const double params[5], double S0, double r, uint32_t flags, uint3
               uint64_t path_offset, uint64_t seed, uint32_t steps, uint32_t n_exp, const double* T,
               const uint32_t* n_strikes, const double* strikes, double* prices, double* std_errs, double* finals,
               uint64_t* counts, int n_threads) {
-    if (n_paths == 0 || steps == 0 || n_exp == 0 || !(payoffs & (ORC_ASIAN | ORC_EUROPEAN))) return -1;
+    if (n_paths == 0 || steps == 0 || n_exp == 0 || !(payoffs & (ORC_ASIAN | ORC_EUROPEAN | ORC_ASIAN_GEO_CV)) ||
+        (payoffs & ~7u))
+        return -1;
     for (uint32_t c = 0; c < n_exp; ++c)
         if (!(T[c] > 0.0) || (c > 0 && !(T[c] > T[c - 1]))) return -2;
     uint32_t* opt_off = (uint32_t*)malloc(sizeof(uint32_t) * (n_exp + 1));
@@ -522,12 +553,25 @@ int orc_price(const double params[5], double S0, double r, uint32_t flags, uint3
         free(opt_off);
         return -1;
     }
-    const int n_pay = ((payoffs & ORC_ASIAN) ? 1 : 0) + ((payoffs & ORC_EUROPEAN) ? 1 : 0);
+    const int n_pay = ((payoffs & ORC_ASIAN) ? 1 : 0) + ((payoffs & ORC_EUROPEAN) ? 1 : 0) +
+                      ((payoffs & ORC_ASIAN_GEO_CV) ? 1 : 0);
+    double* geo_exact = NULL;
+    if (payoffs & ORC_ASIAN_GEO_CV) {
+        geo_exact = (double*)malloc(sizeof(double) * n_opts);
+        if (!geo_exact ||
+            orc_geo_asian_price(params, S0, (flags & ORC_DRIFT) ? r : 0.0, T, n_exp, n_strikes, steps, flags, strikes,
+                                geo_exact) < 0) {
+            free(geo_exact);
+            free(opt_off);
+            return -3;
+        }
+    }
     const uint64_t n_out = n_opts * (uint64_t)n_pay;
     const uint64_t n_blocks = (n_paths + ORC_BLOCK - 1) / ORC_BLOCK;
     acc2* part = (acc2*)calloc((size_t)(n_blocks * n_out * 2), sizeof(acc2));
     uint64_t cq = 0, ce = 0;
     if (!part) {
+        free(geo_exact);
         free(opt_off);
         return -3;
     }
@@ -573,9 +617,12 @@ int orc_price(const double params[5], double S0, double r, uint32_t flags, uint3
         const double mean = acc_val(&s) / n;
         double var = n > 1 ? (acc_val(&ss) - n * mean * mean) / (n - 1.0) : 0.0;
         if (var < 0) var = 0;
-        prices[o] = disc * mean;
+        /* the control-variate outputs are the last n_opts */
+        const double exact = (geo_exact && o >= n_out - n_opts) ? geo_exact[opt] : 0.0;
+        prices[o] = disc * (mean + exact);
         if (std_errs) std_errs[o] = disc * sqrt(var / n);
     }
+    free(geo_exact);
     free(part);
     free(opt_off);
     return 0;

@@ -40,6 +40,7 @@ extern "C" {
 #define ORC_DISCOUNT 2u  /* multiply the estimator by exp(-r T) (extension) */
 #define ORC_TRAPEZOID 4u /* corrected trapezoid Asian average (extension) */
 #define ORC_MARTINGALE 8u /* Andersen's martingale-corrected K0 (extension) */
+#define ORC_ASIAN_GEO_CV 4u /* payoff: Asian call with the geometric-average control variate (extension) */
 
 /* ---- Philox4x32-10 ---- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
@@ -77,7 +78,9 @@ int64_t orc_time_grid(const double* T, uint32_t n_exp, const uint32_t* n_strikes
 
 /* ---- Monte-Carlo price (reference semantics, FP64) ----
  * Prices every option of every chain. Outputs are payoff-major: for each payoff present in
- * `payoffs` (Asian first, then European), n_opts entries in chain-major order.
+ * `payoffs` (Asian, European, then the Asian with the geometric control variate), n_opts
+ * entries in chain-major order. The control-variate price is mean(A payoff - G payoff) plus the
+ * exact geometric price (orc_geo_asian_price), times the discount factor.
  * finals (nullable): [n_paths][n_exp][2] = (Asian average A_c, interpolated X_T) per path.
  * counts (nullable): [2] = (quadratic-branch steps, exponential-branch steps).
  * n_threads: OpenMP threads (0 = library default). Returns 0 or a negative error. */
@@ -91,6 +94,27 @@ void orc_qe_moments(const double params[5], double delta, double V, uint64_t n, 
 /* Wall-clock of the last orc_price call (seconds, steady clock, pricing loop only). */
 double orc_last_seconds(void);
 int orc_max_threads(void);
+
+/* ---- geometric-average control variate (geo_oracle.c) ----
+ * orc_asian_weights: weights w[0..n] (and times t[0..n]) of chain `chain`'s Asian functional
+ * A = sum_j w_j X_{t_j} (src/HSimulation.tpp:30-45, src/inc/AsianContract.h:23-38); returns n,
+ * -2 if the chain is never finalised. orc_geo_cf: E[exp(s sum_j v_j ln(X_{t_j}/S0))] under
+ * Heston. orc_geo_price_weights: E[(c e^L - K)^+] with c = W S0, L = sum_j (w_j/W) ln(X_{t_j}/S0),
+ * W = sum_j w_j (undiscounted). orc_geo_asian_price: that price for every option of a request
+ * (chain-major). */
+int64_t orc_asian_weights(const double* T, uint32_t n_exp, const uint32_t* n_strikes, uint32_t steps, uint32_t flags,
+                          uint32_t chain, double* t, double* w, uint64_t cap);
+#ifndef __cplusplus
+double _Complex orc_geo_cf(const double params[5], double r, const double* t, const double* v, int64_t n,
+                           double _Complex s);
+#endif
+void orc_geo_cf_out(const double params[5], double r, const double* t, const double* v, int64_t n, double s_re,
+                    double s_im, double out[2]);
+int orc_geo_price_weights(const double params[5], double S0, double r, const double* t, const double* w, int64_t n,
+                          uint32_t n_strikes, const double* strikes, double* out);
+int orc_geo_asian_price(const double params[5], double S0, double r, const double* T, uint32_t n_exp,
+                        const uint32_t* n_strikes, uint32_t steps, uint32_t flags, const double* strikes,
+                        double* out);
 
 /* ---- SWIFT European oracle (swift_oracle.c) ----
  * Restates src/SWIFT.cpp:18-118 and src/HDistribution.cpp:8-38,86-113.

@@ -16,7 +16,7 @@ import platform
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ASIAN, EUROPEAN = 1, 2
+ASIAN, EUROPEAN, ASIAN_GEO_CV = 1, 2, 4
 DRIFT, DISCOUNT, TRAPEZOID, MARTINGALE = 1, 2, 4, 8
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -49,6 +49,16 @@ def _bind(lib: C.CDLL) -> C.CDLL:
                               C.c_uint64, C.c_uint32, C.c_uint32, _f64p, _u32p, _f64p, _f64p, _f64p, C.c_void_p,
                               C.c_void_p, C.c_int]
     lib.orc_price.restype = C.c_int
+    lib.orc_asian_weights.argtypes = [_f64p, C.c_uint32, _u32p, C.c_uint32, C.c_uint32, C.c_uint32, _f64p, _f64p,
+                                      C.c_uint64]
+    lib.orc_asian_weights.restype = C.c_int64
+    lib.orc_geo_cf_out.argtypes = [_f64p, C.c_double, _f64p, _f64p, C.c_int64, C.c_double, C.c_double, _f64p]
+    lib.orc_geo_price_weights.argtypes = [_f64p, C.c_double, C.c_double, _f64p, _f64p, C.c_int64, C.c_uint32,
+                                          _f64p, _f64p]
+    lib.orc_geo_price_weights.restype = C.c_int
+    lib.orc_geo_asian_price.argtypes = [_f64p, C.c_double, C.c_double, _f64p, C.c_uint32, _u32p, C.c_uint32,
+                                        C.c_uint32, _f64p, _f64p]
+    lib.orc_geo_asian_price.restype = C.c_int
     lib.orc_last_seconds.restype = C.c_double
     lib.orc_max_threads.restype = C.c_int
     return lib
@@ -183,7 +193,7 @@ def price(params, S0: float, T, n_strikes, strikes, *, steps: int, n_paths: int,
     T = np.ascontiguousarray(np.atleast_1d(T), np.float64)
     ns = np.ascontiguousarray(np.atleast_1d(n_strikes), np.uint32)
     K = np.ascontiguousarray(np.atleast_1d(strikes), np.float64)
-    n_pay = (1 if payoffs & ASIAN else 0) + (1 if payoffs & EUROPEAN else 0)
+    n_pay = (1 if payoffs & ASIAN else 0) + (1 if payoffs & EUROPEAN else 0) + (1 if payoffs & ASIAN_GEO_CV else 0)
     n_out = int(ns.sum()) * n_pay
     prices = np.zeros(n_out, np.float64)
     ses = np.zeros(n_out, np.float64)
@@ -200,6 +210,55 @@ def price(params, S0: float, T, n_strikes, strikes, *, steps: int, n_paths: int,
         out["finals"] = finals
     if counts is not None:
         out["counts"] = counts
+    return out
+
+
+def asian_weights(T, n_strikes, steps: int, chain: int, flags: int = 0):
+    """(t[0..n], w[0..n]): grid times and weights of chain `chain`'s Asian functional
+    A = sum_j w_j X_{t_j} under the reference's time loop."""
+    T = np.ascontiguousarray(np.atleast_1d(T), np.float64)
+    ns = np.ascontiguousarray(np.atleast_1d(n_strikes), np.uint32)
+    cap = 4 * steps * len(T) + 64
+    t = np.zeros(cap, np.float64)
+    w = np.zeros(cap, np.float64)
+    n = load().orc_asian_weights(T, len(T), ns, steps, flags, chain, t, w, cap)
+    if n < 0:
+        raise ValueError(f"orc_asian_weights failed: {n}")
+    return t[:n + 1].copy(), w[:n + 1].copy()
+
+
+def geo_cf(params, r: float, t, v, s: complex) -> complex:
+    """E[exp(s sum_j v_j ln(X_{t_j}/S0))] under Heston (t[0] = 0; v[0] is ignored)."""
+    t = np.ascontiguousarray(t, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    out = np.zeros(2, np.float64)
+    load().orc_geo_cf_out(np.asarray(params, np.float64), r, t, v, len(t) - 1, s.real, s.imag, out)
+    return complex(out[0], out[1])
+
+
+def geo_price_weights(params, S0: float, r: float, t, w, strikes) -> np.ndarray:
+    """E[(W S0 exp(sum_j (w_j/W) ln(X_{t_j}/S0)) - K)^+], W = sum_j w_j, undiscounted."""
+    t = np.ascontiguousarray(t, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    K = np.ascontiguousarray(np.atleast_1d(strikes), np.float64)
+    out = np.zeros(len(K), np.float64)
+    rc = load().orc_geo_price_weights(np.asarray(params, np.float64), S0, r, t, w, len(t) - 1, len(K), K, out)
+    if rc not in (0, -4):
+        raise ValueError(f"orc_geo_price_weights failed: {rc}")
+    return out
+
+
+def geo_asian_price(params, S0: float, T, n_strikes, strikes, *, steps: int, r: float = 0.0,
+                    flags: int = 0) -> np.ndarray:
+    """Exact price of the geometric counterpart of every option's Asian functional (undiscounted;
+    r is the drift, as with the DRIFT flag)."""
+    T = np.ascontiguousarray(np.atleast_1d(T), np.float64)
+    ns = np.ascontiguousarray(np.atleast_1d(n_strikes), np.uint32)
+    K = np.ascontiguousarray(np.atleast_1d(strikes), np.float64)
+    out = np.zeros(len(K), np.float64)
+    rc = load().orc_geo_asian_price(np.asarray(params, np.float64), S0, r, T, len(T), ns, steps, flags, K, out)
+    if rc != 0:
+        raise ValueError(f"orc_geo_asian_price failed: {rc}")
     return out
 
 

@@ -1,0 +1,131 @@
+"""CPU oracle: the geometric-average control variate for the Asian call (payoff ASIAN_GEO_CV).
+
+Pins (none of them restates the oracle's formulas):
+ * Asian functional weights: hand-derived weights of the reference's time loop for small grids
+   (both schedule cases, the corrected trapezoid, a second expiry after a step-width change),
+   and E[A] = sum_j w_j S0 e^{r t_j} for K = 0 under the martingale-corrected scheme;
+ * exact geometric price: sigma -> 0 with v0 = theta makes L Gaussian with a closed-form mean and
+   variance (a Black-type formula); a functional on one date is the Heston European call, which
+   the SWIFT oracle prices (pinned by the reference's golden prices);
+ * the estimator: on fine grids the control-variate price agrees with the plain Monte-Carlo
+   Asian price within the plain estimator's standard error, with a far smaller standard error.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.stats import norm
+
+from oracle import oracle as O
+
+
+def test_weights_hand_derived_case_a():
+    # T = 1, N = 2: points t = 0, .5, 1; one accumulation dt/2 (X1 + X0), then the quirk
+    # (X2 - X1) (T - t1)/dt with (T - t1)/dt = 1; divided by T
+    t, w = O.asian_weights([1.0], [1], 2, 0)
+    np.testing.assert_allclose(t, [0.0, 0.5, 1.0])
+    np.testing.assert_allclose(w, [0.25, 0.25 - 1.0, 1.0], atol=1e-15)
+    t, w = O.asian_weights([1.0], [1], 2, 0, flags=O.TRAPEZOID)
+    # trapezoid end: (T - t1)/2 (X1 + X_T) with X_T = X2 here
+    np.testing.assert_allclose(w, [0.25, 0.25 + 0.25, 0.25], atol=1e-15)
+
+
+def test_weights_hand_derived_second_expiry():
+    # T = (0.5, 1), N = 2: dt 0.25 up to the first expiry (finalised at t = 0.5, 2 steps), then
+    # dt = 0.5: the accumulation right after the first finalisation uses the new dt
+    t, w = O.asian_weights([0.5, 1.0], [1, 1], 2, 1)
+    np.testing.assert_allclose(t, [0.0, 0.25, 0.5, 1.0])
+    acc = np.array([0.125, 0.125 + 0.25, 0.25, 0.0])   # 0.25/2 (X1+X0) + 0.5/2 (X2+X1)
+    quirk = np.array([0.0, 0.0, -1.0, 1.0])             # (X3 - X2) (1 - 0.5)/0.5
+    np.testing.assert_allclose(w, acc + quirk, atol=1e-15)
+
+
+@pytest.mark.parametrize("T,N", [(1.0, 12), (1.0, 252), (0.1, 5), (3.0, 156)])
+def test_weight_sum_is_the_k0_invariant(T, N):
+    """sum_j w_j is E[A]/S0 at K = 0 without drift: (N-1)/N when the last step lands on T (the
+    reference's schedule case A), else 1."""
+    t, w = O.asian_weights([T], [1], N, 0)
+    _, fs, _ = O.time_grid([T], [1], N)
+    expect = (N - 1) / N if int(fs[0]) == N else 1.0
+    assert abs(w.sum() - expect) < 1e-12
+
+
+def test_weights_give_the_mc_mean_of_a_at_k0():
+    params, S0, r, T, N = [0.04, 0.04, -0.7, 2.0, 0.3], 100.0, 0.05, [0.5, 1.3], 7
+    res = O.price(params, S0, T, [1, 1], [0.0, 0.0], steps=N, n_paths=400_000, seed=3, r=r,
+                  flags=O.DRIFT | O.MARTINGALE, payoffs=O.ASIAN)
+    for c in range(2):
+        t, w = O.asian_weights(T, [1, 1], N, c)
+        expect = float(np.sum(w * S0 * np.exp(r * t)))
+        assert abs(res["prices"][c] - expect) < 4 * res["std_errs"][c], (c, res["prices"][c], expect)
+
+
+def gbm_geo(S0, r, th, t, w, K):
+    W = w.sum()
+    v = w / W
+    S = np.cumsum(v[::-1])[::-1]          # suffix sums: weight still to come at each point
+    mu = float(np.sum(np.diff(t) * (r - 0.5 * th) * S[1:]))
+    var = float(th * np.sum(np.diff(t) * S[1:] ** 2))
+    c = W * S0
+    k = np.log(np.asarray(K) / c)
+    sd = math.sqrt(var)
+    d1 = (mu + var - k) / sd
+    return c * (np.exp(mu + 0.5 * var) * norm.cdf(d1) - np.exp(k) * norm.cdf(d1 - sd))
+
+
+@pytest.mark.parametrize("flags", [0, O.TRAPEZOID])
+@pytest.mark.parametrize("T,N,chain", [([1.0], 12, 0), ([0.5, 1.0, 2.0], 26, 2), ([0.1], 5, 0), ([2.0], 3, 0)])
+def test_gbm_limit_closed_form(flags, T, N, chain):
+    th, r, S0 = 0.04, 0.02, 100.0
+    t, w = O.asian_weights(T, [1] * len(T), N, chain, flags)
+    K = np.array([70.0, 90.0, 100.0, 110.0, 140.0])
+    g = O.geo_price_weights([th, th, -0.7, 2.0, 1e-9], S0, r, t, w, K)
+    np.testing.assert_allclose(g, gbm_geo(S0, r, th, t, w, K), rtol=0, atol=1e-9 * S0)
+
+
+@pytest.mark.parametrize("params,T", [
+    ([0.04, 0.04, -0.7, 2.0, 0.3], 1.0),
+    ([0.04, 0.04, -0.9, 0.5, 0.9], 1.0),
+    ([0.01, 0.01, -0.5, 0.5, 1.0], 0.1),    # peaked density: slow characteristic-function decay
+    ([0.2, 0.1, -0.3, 5.0, 0.5], 5.0),
+    ([0.04, 0.09, 0.4, 1.5, 0.4], 2.0),     # positive correlation
+])
+def test_one_date_is_the_heston_european(params, T):
+    t = np.linspace(0.0, T, 17)
+    w = np.zeros(17)
+    w[-1] = 1.0
+    K = np.array([80.0, 100.0, 120.0])
+    g = O.geo_price_weights(params, 100.0, 0.02, t, w, K)
+    sw = O.swift_price(params, 100.0, 0.02, T, K, trunc=1e-12)[0] * math.exp(0.02 * T)
+    np.testing.assert_allclose(g, sw, rtol=0, atol=1e-9 * 100.0)
+
+
+@pytest.mark.parametrize("params", [[0.04, 0.04, -0.7, 2.0, 0.3], [0.04, 0.04, -0.9, 0.5, 0.9]])
+def test_control_variate_estimator(params):
+    """Fine grid (200 steps) with the martingale correction: the control-variate price equals
+    the plain Asian price within the plain standard error (their difference is the exact
+    geometric price minus its Monte-Carlo estimate), and its standard error is far smaller."""
+    S0, r, T = 100.0, 0.02, [1.0]
+    K = [80.0, 100.0, 120.0]
+    res = O.price(params, S0, T, [3], K, steps=200, n_paths=200_000, seed=17, r=r,
+                  flags=O.DRIFT | O.MARTINGALE, payoffs=O.ASIAN | O.ASIAN_GEO_CV)
+    plain, cv = res["prices"][:3], res["prices"][3:]
+    se_plain, se_cv = res["std_errs"][:3], res["std_errs"][3:]
+    assert np.all(np.abs(cv - plain) < 4 * se_plain + 1e-3), (cv - plain, se_plain)
+    # in and at the money the two payoffs move together; deep out of the money less so
+    assert np.all(se_cv[:2] < se_plain[:2] / 10) and se_cv[2] < se_plain[2] / 2, (se_cv, se_plain)
+
+
+def test_control_variate_output_order_and_exact_part():
+    params, S0, T, K = [0.04, 0.04, -0.7, 2.0, 0.3], 100.0, [0.5, 1.0], [95.0, 100.0, 105.0]
+    kw = dict(steps=13, n_paths=5000, seed=2, r=0.02, flags=O.DRIFT | O.DISCOUNT)
+    both = O.price(params, S0, T, [2, 1], K, payoffs=O.ASIAN | O.EUROPEAN | O.ASIAN_GEO_CV, **kw)
+    alone = O.price(params, S0, T, [2, 1], K, payoffs=O.ASIAN_GEO_CV, **kw)
+    np.testing.assert_array_equal(both["prices"][6:], alone["prices"])
+    asian = O.price(params, S0, T, [2, 1], K, payoffs=O.ASIAN, **kw)
+    np.testing.assert_array_equal(both["prices"][:3], asian["prices"])
+    # the CV price minus the discounted exact geometric price is a plain mean of a - g
+    geo = O.geo_asian_price(params, S0, T, [2, 1], K, steps=13, r=0.02)
+    disc = np.exp(-0.02 * np.array([0.5, 0.5, 1.0]))
+    diff = alone["prices"] - disc * geo
+    assert np.all(np.abs(diff) < 1.0)   # a - g is small per path: mean of the difference
