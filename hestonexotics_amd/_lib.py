@@ -16,7 +16,7 @@ ERRORS = {
     -5: "HEXO_ERR_ADMISSION", -6: "HEXO_ERR_NOT_FOUND", -7: "HEXO_ERR_CANCELLED", -8: "HEXO_ERR_TIMEOUT",
     -9: "HEXO_ERR_STATE", -10: "HEXO_ERR_INJECTED",
 }
-ASIAN, EUROPEAN = 1, 2
+ASIAN, EUROPEAN, ASIAN_GEO_CV = 1, 2, 4
 DRIFT, DISCOUNT, TRAPEZOID, MARTINGALE = 1, 2, 4, 8
 FP64, FP32, FP32_COMPENSATED = 0, 1, 2
 
@@ -109,6 +109,7 @@ SIGNATURES = {
     "hexo_validate_request": (C.c_int, [_req]),
     "hexo_request_n_outputs": (C.c_uint64, [_req]),
     "hexo_time_grid": (C.c_int, [_d, C.c_uint32, _u32, C.c_uint32, _u64, _d, _u64]),
+    "hexo_asian_weights": (C.c_int64, [_req, C.c_uint32, _d, _d, C.c_uint64]),
     "hexo_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "hexo_ctx_destroy": (C.c_int, [_P]),
     "hexo_ctx_last_error": (C.c_char_p, [_P]),
@@ -124,6 +125,7 @@ SIGNATURES = {
     "hexo_batch_branch_counts": (C.c_int, [_P, _u64]),
     "hexo_batch_destroy": (C.c_int, [_P]),
     "hexo_paths": (C.c_int, [_P, _req, C.c_uint64, C.c_uint64, _d]),
+    "hexo_geo_asian_price": (C.c_int, [_P, _req, _d]),
     "hexo_rng_fill": (C.c_int, [_P, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, _u32]),
     "hexo_philox4x32_10": (C.c_int, [_P, _u32, _u32, C.c_uint64, _u32]),
     "hexo_normals": (C.c_int, [_P, _u32, C.c_uint64, C.c_uint32, _d]),

@@ -16,14 +16,14 @@ from typing import Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import (ASIAN, CALIB_INITIAL_GIVEN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED, FP64,
-                   MARTINGALE, SWIFT_FIXED_GRID, TRAPEZOID, HexoError, hexo_batch_stats, hexo_calib_chain,
+from ._lib import (ASIAN, ASIAN_GEO_CV, CALIB_INITIAL_GIVEN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED,
+                   FP64, MARTINGALE, SWIFT_FIXED_GRID, TRAPEZOID, HexoError, hexo_batch_stats, hexo_calib_chain,
                    hexo_calib_config, hexo_calib_result, hexo_request, hexo_server_config, hexo_server_stats,
                    hexo_swift_grid, hexo_swift_request, hexo_ticket_info, hexo_tuning)
 
-__all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "DRIFT", "DISCOUNT", "TRAPEZOID", "MARTINGALE",
-           "FP64", "FP32", "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "SwiftChain", "SwiftCall",
-           "swift_grid"]
+__all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "ASIAN_GEO_CV", "DRIFT", "DISCOUNT", "TRAPEZOID",
+           "MARTINGALE", "FP64", "FP32", "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "asian_weights",
+           "SwiftChain", "SwiftCall", "swift_grid"]
 
 GRID_FIELDS = ("m", "exp2_m", "sqrt_exp2_m", "lower", "upper", "k1", "k2", "J")
 
@@ -103,7 +103,7 @@ class Request:
 
     @property
     def n_payoffs(self) -> int:
-        return (1 if self.payoffs & ASIAN else 0) + (1 if self.payoffs & EUROPEAN else 0)
+        return bin(self.payoffs & (ASIAN | EUROPEAN | ASIAN_GEO_CV)).count("1")
 
     @property
     def n_outputs(self) -> int:
@@ -135,14 +135,14 @@ class Request:
         return q
 
     def split(self, prices: np.ndarray) -> dict:
-        """Outputs are payoff-major: the Asian block (if priced) then the European block."""
+        """Outputs are payoff-major: the Asian, European and control-variate Asian blocks, those
+        priced, in that order."""
         out, o = {}, 0
         n = self.n_opts
-        if self.payoffs & ASIAN:
-            out["asian"] = prices[o:o + n]
-            o += n
-        if self.payoffs & EUROPEAN:
-            out["european"] = prices[o:o + n]
+        for bit, name in ((ASIAN, "asian"), (EUROPEAN, "european"), (ASIAN_GEO_CV, "asian_geo_cv")):
+            if self.payoffs & bit:
+                out[name] = prices[o:o + n]
+                o += n
         return out
 
 
@@ -160,6 +160,19 @@ def _dp(a: np.ndarray):
 def validate(req: Request) -> int:
     lib = _lib.load()
     return lib.hexo_validate_request(C.byref(req.to_c()))
+
+
+def asian_weights(req: Request, chain: int):
+    """(t, w): grid times and weights of expiry `chain`'s Asian average A = sum_j w_j X(t_j), as
+    the path kernels accumulate it (host only)."""
+    lib = _lib.load()
+    cap = 4 * int(req.steps) * len(req.expiries) + 64
+    t = np.zeros(cap, np.float64)
+    w = np.zeros(cap, np.float64)
+    n = lib.hexo_asian_weights(C.byref(req.to_c()), chain, _dp(t), _dp(w), cap)
+    if n < 0:
+        raise HexoError(int(n), "asian weights")
+    return t[:n + 1].copy(), w[:n + 1].copy()
 
 
 def time_grid(expiries, n_strikes, steps: int):
@@ -292,6 +305,12 @@ class Context:
     def paths(self, req: Request, first: int, n: int) -> np.ndarray:
         out = np.zeros((n, len(req.expiries), 2), np.float64)
         self._check(self.lib.hexo_paths(self.h, C.byref(req.to_c()), first, n, _dp(out.reshape(-1))))
+        return out
+
+    def geo_asian_price(self, req: Request) -> np.ndarray:
+        """Exact Heston price of each option's geometric control variate (chain-major)."""
+        out = np.zeros(req.n_opts, np.float64)
+        self._check(self.lib.hexo_geo_asian_price(self.h, C.byref(req.to_c()), _dp(out)))
         return out
 
     # ---- components ----

@@ -309,6 +309,36 @@ int hexo_paths(hexo_ctx* ctx, const hexo_request* req, uint64_t first, uint64_t 
     return HEXO_OK;
 }
 
+int hexo_geo_asian_price(hexo_ctx* ctx, const hexo_request* req, double* prices) {
+    if (!ctx) return no_ctx();
+    if (!req || !prices) return ctx->set_error(HEXO_ERR_INVALID, "request and prices must be non-NULL");
+    // a batch of the request's control-variate block alone: creating it computes the exact prices
+    hexo_request q = *req;
+    q.payoffs = HEXO_ASIAN_GEO_CV;
+    q.n_paths = hexo::kTile;
+    q.path_offset = 0;
+    std::string why = hexo::validate(&q);
+    if (!why.empty()) return ctx->set_error(HEXO_ERR_INVALID, why);
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    DeviceGuard dg(ctx->device);
+    std::string err;
+    std::unique_ptr<hexo::Batch> b;
+    int rc = hexo::batch_create(ctx, &q, 1, ctx->stream, nullptr, b, err);
+    if (rc != HEXO_OK) return ctx->set_error(rc, err);
+    const uint64_t n_opts = hexo::n_opts_of(&q);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        hipMemcpy(prices, b->ga.geo, sizeof(double) * n_opts, hipMemcpyDeviceToHost) != hipSuccess)
+        return ctx->set_error(HEXO_ERR_HIP, "exact geometric prices failed");
+    if (q.flags & HEXO_DISCOUNT) {
+        uint64_t o = 0;
+        for (uint32_t c = 0; c < q.n_expiries; ++c) {
+            const double disc = std::exp(-q.r * q.expiries[c]);
+            for (uint32_t j = 0; j < q.n_strikes[c]; ++j) prices[o++] *= disc;
+        }
+    }
+    return HEXO_OK;
+}
+
 int hexo_rng_fill(hexo_ctx* ctx, uint64_t seed, uint32_t stream, uint64_t path0, uint64_t n_paths, uint64_t word0,
                   uint64_t n_words, uint32_t* out) {
     if (!ctx) return no_ctx();

@@ -24,7 +24,10 @@ double now_s() {
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-uint32_t n_payoffs(uint32_t payoffs) { return ((payoffs & HEXO_ASIAN) ? 1u : 0u) + ((payoffs & HEXO_EUROPEAN) ? 1u : 0u); }
+uint32_t n_payoffs(uint32_t payoffs) {
+    return ((payoffs & HEXO_ASIAN) ? 1u : 0u) + ((payoffs & HEXO_EUROPEAN) ? 1u : 0u) +
+           ((payoffs & HEXO_ASIAN_GEO_CV) ? 1u : 0u);
+}
 
 uint64_t n_opts_of(const hexo_request* r) {
     uint64_t n = 0;
@@ -110,7 +113,8 @@ std::string validate(const hexo_request* r) {
     if (!r->strikes) return "strikes must be non-NULL";
     for (uint64_t i = 0; i < n_opts; ++i)
         if (!fin(r->strikes[i])) return "strikes must be finite";
-    if (r->payoffs == 0 || (r->payoffs & ~3u)) return "payoffs must be a non-empty subset of ASIAN|EUROPEAN";
+    if (r->payoffs == 0 || (r->payoffs & ~7u))
+        return "payoffs must be a non-empty subset of ASIAN|EUROPEAN|ASIAN_GEO_CV";
     if (r->flags & ~15u) return "unknown flag bits";
     if (r->precision > 2u) return "precision must be HEXO_FP64, HEXO_FP32 or HEXO_FP32_COMPENSATED";
     const uint64_t n_tiles = (r->n_paths + kTile - 1) / kTile;
@@ -119,6 +123,11 @@ std::string validate(const hexo_request* r) {
     uint64_t total = 0;
     if (!build_schedule(r->expiries, r->n_expiries, r->n_strikes, r->steps, segs, total))
         return "time grid could not be built";
+    if (r->payoffs & HEXO_ASIAN_GEO_CV) {
+        GeoPlan g;
+        std::string why;
+        if (!plan_geo(r, segs, g, why)) return why;
+    }
     return "";
 }
 
@@ -197,6 +206,8 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct ReqPlan {
     std::vector<HostSeg> segs;
+    GeoPlan geo;
+    std::string geo_err;
     uint64_t total_steps = 0, n_opts = 0, n_out = 0;
     uint32_t n_pay = 0, n_tiles = 0, n_groups = 0, cls = 0;
     uint32_t n_fins = 0;
@@ -205,12 +216,14 @@ struct ReqPlan {
 struct Layout {
     size_t reqs, segs, fins, opts, order[kPathClasses], item_end[kPathClasses], group_end, out_end;  // uploaded
     size_t upload_end;
-    size_t counters, counts, partials, groups, prices, ses, finals;  // device only
+    size_t geo_items, geo_pts, gl;                                  // uploaded
+    size_t counters, counts, partials, groups, prices, ses, finals, geo;  // device only
     size_t total;
 };
 
 Layout make_layout(uint32_t n_req, uint64_t n_seg, uint64_t n_fin, uint64_t n_opt, const uint32_t* n_cls,
-                   uint64_t n_partials, uint64_t n_groups, uint64_t n_out, uint64_t n_finals) {
+                   uint64_t n_partials, uint64_t n_groups, uint64_t n_out, uint64_t n_finals, uint64_t n_geo_items,
+                   uint64_t n_geo_pts) {
     Layout L{};
     size_t o = 0;
     auto take = [&](size_t bytes) {
@@ -228,6 +241,9 @@ Layout make_layout(uint32_t n_req, uint64_t n_seg, uint64_t n_fin, uint64_t n_op
     }
     L.group_end = take(sizeof(uint64_t) * n_req);
     L.out_end = take(sizeof(uint64_t) * n_req);
+    L.geo_items = take(sizeof(DevGeo) * std::max<uint64_t>(n_geo_items, 1));
+    L.geo_pts = take(2 * sizeof(double) * std::max<uint64_t>(n_geo_pts, 1));
+    L.gl = take(128 * sizeof(double));
     L.upload_end = o;
     L.counters = take(256);
     L.counts = take(256);
@@ -236,6 +252,7 @@ Layout make_layout(uint32_t n_req, uint64_t n_seg, uint64_t n_fin, uint64_t n_op
     L.prices = take(sizeof(double) * n_out);
     L.ses = take(sizeof(double) * n_out);
     L.finals = take(sizeof(double) * std::max<uint64_t>(n_finals, 1));
+    L.geo = take(sizeof(double) * std::max<uint64_t>(n_out, 1));
     L.total = o;
     return L;
 }
@@ -247,7 +264,10 @@ bool plan_request(const hexo_request* r, ReqPlan& p) {
     p.n_out = p.n_opts * p.n_pay;
     p.n_tiles = (uint32_t)((r->n_paths + kTile - 1) / kTile);
     p.n_groups = (p.n_tiles + kGroupTiles - 1) / kGroupTiles;
-    p.cls = r->precision == HEXO_FP64 ? 0u : (r->flags & HEXO_MARTINGALE) ? 2u : 1u;
+    p.cls = r->precision == HEXO_FP64
+                ? 0u
+                : 1u + ((r->flags & HEXO_MARTINGALE) ? 1u : 0u) + ((r->payoffs & HEXO_ASIAN_GEO_CV) ? 2u : 0u);
+    if (!plan_geo(r, p.segs, p.geo, p.geo_err) && p.geo_err.empty()) p.geo_err = "geometric control variate";
     p.n_fins = 0;
     for (auto& s : p.segs) p.n_fins += (uint32_t)s.fins.size();
     return true;
@@ -259,11 +279,12 @@ size_t request_device_bytes(const hexo_request* r) {
     ReqPlan p;
     if (!plan_request(r, p)) return 0;
     return sizeof(DevReq) + p.segs.size() * sizeof(DevSeg) + p.n_fins * sizeof(DevFin) + p.n_opts * sizeof(DevOpt) +
-           32 + 16 + (size_t)16 * p.n_tiles * p.n_out + (size_t)16 * p.n_groups * p.n_out + 16 * p.n_out;
+           32 + 16 + (size_t)16 * p.n_tiles * p.n_out + (size_t)16 * p.n_groups * p.n_out + 24 * p.n_out +
+           p.geo.items.size() * sizeof(DevGeo) + p.geo.pts.size() * sizeof(double);
 }
 
 size_t batch_device_bytes(const hexo_request* reqs, uint32_t n) {
-    size_t s = 24 * 256;  // alignment slack of the region table
+    size_t s = 40 * 256 + 128 * sizeof(double);  // alignment slack of the region table, quadrature table
     for (uint32_t i = 0; i < n; ++i) s += request_device_bytes(&reqs[i]);
     return s;
 }
@@ -293,7 +314,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         return HEXO_ERR_INVALID;
     }
     std::vector<ReqPlan> plans(n);
-    uint64_t n_seg = 0, n_fin = 0, n_opt = 0, n_partials = 0, n_groups = 0, n_out = 0;
+    uint64_t n_seg = 0, n_fin = 0, n_opt = 0, n_partials = 0, n_groups = 0, n_out = 0, n_geo_items = 0, n_geo_pts = 0;
     uint32_t n_cls[kPathClasses] = {};
     for (uint32_t i = 0; i < n; ++i) {
         std::string why = validate(&reqs[i]);
@@ -306,6 +327,12 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
             return HEXO_ERR_INVALID;
         }
         const ReqPlan& p = plans[i];
+        if (!p.geo_err.empty()) {
+            err = "request " + std::to_string(i) + ": " + p.geo_err;
+            return HEXO_ERR_INVALID;
+        }
+        n_geo_items += p.geo.items.size();
+        n_geo_pts += p.geo.pts.size() / 2;
         n_seg += p.segs.size();
         n_fin += p.n_fins;
         n_opt += p.n_opts;
@@ -320,9 +347,10 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     b->n_req = n;
     b->n_out_total = n_out;
     b->ppt[0] = 1;
-    b->ppt[1] = b->ppt[2] = ctx->tuning.paths_per_thread;
+    for (uint32_t cls = 1; cls < kPathClasses; ++cls) b->ppt[cls] = ctx->tuning.paths_per_thread;
     const uint64_t n_finals = finals_n ? *finals_n : 0;
-    const Layout L = make_layout(n, n_seg, n_fin, n_opt, n_cls, n_partials, n_groups, n_out, n_finals);
+    const Layout L =
+        make_layout(n, n_seg, n_fin, n_opt, n_cls, n_partials, n_groups, n_out, n_finals, n_geo_items, n_geo_pts);
 
     // host staging: the uploaded tables, then the result buffers
     b->upload_bytes = L.upload_end;
@@ -350,6 +378,10 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     DevOpt* ho = reinterpret_cast<DevOpt*>(H + L.opts);
     uint64_t* hge = reinterpret_cast<uint64_t*>(H + L.group_end);
     uint64_t* hoe = reinterpret_cast<uint64_t*>(H + L.out_end);
+    DevGeo* hgi = reinterpret_cast<DevGeo*>(H + L.geo_items);
+    double* hgp = reinterpret_cast<double*>(H + L.geo_pts);
+    gauss_legendre64(reinterpret_cast<double*>(H + L.gl), reinterpret_cast<double*>(H + L.gl) + 64);
+    uint64_t geo_i = 0, geo_p = 0;
 
     const double L2E = 1.4426950408889634074;  // log2(e)
     uint64_t seg_i = 0, fin_i = 0, opt_i = 0, part_i = 0, grp_i = 0, out_i = 0, ge = 0;
@@ -399,6 +431,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
             }
         }
         // runs of equal steps with their FP32 constants (computed in FP64)
+        size_t fin_local = 0;  // index into the request's per-finalisation W
         for (const HostSeg& S : p.segs) {
             DevSeg& g = hs[seg_i++];
             const double dl = S.delta, k = q.kappa, th = q.theta, e = q.sigma, rho = q.rho;
@@ -438,7 +471,30 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
                 f.T = F.T;
                 f.tp = F.tp;
                 f.dt = F.dt;
+                const double W = p.geo.fin_W[fin_local++];
+                f.gW = W;
+                f.gW_f = (float)W;
+                f.gInvW_f = (float)(1.0 / W);
             }
+        }
+        // exact-price items of the geometric control variate: points rebased into the batch's
+        // list; outputs follow the Asian and European blocks
+        if (!p.geo.items.empty()) {
+            const uint64_t cvpay = n_payoffs(q.payoffs & (HEXO_ASIAN | HEXO_EUROPEAN));
+            for (const HostGeoItem& it : p.geo.items) {
+                DevGeo& g = hgi[geo_i++];
+                g.req = i;
+                g.pts_begin = (uint32_t)(geo_p + it.pts_begin);
+                g.n_pts = it.n_pts;
+                g.n_strikes = it.n_strikes;
+                g.opt = (uint32_t)(opt_i + it.opt);
+                g.max_panels = it.max_panels;
+                g.out = out_i + cvpay * p.n_opts + it.opt;
+                g.c = it.c;
+                g.h = it.h;
+            }
+            std::memcpy(hgp + 2 * geo_p, p.geo.pts.data(), sizeof(double) * p.geo.pts.size());
+            geo_p += p.geo.pts.size() / 2;
         }
         b->out_base[i] = out_i;
         opt_i += p.n_opts;
@@ -518,6 +574,15 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     ReduceArgs& ra = b->ra;
     ra.reqs = reinterpret_cast<const DevReq*>(D + L.reqs);
     ra.opts = reinterpret_cast<const DevOpt*>(D + L.opts);
+    ra.geo = geo_i ? reinterpret_cast<const double*>(D + L.geo) : nullptr;
+    GeoArgs& ga = b->ga;
+    ga.reqs = ra.reqs;
+    ga.opts = ra.opts;
+    ga.items = reinterpret_cast<const DevGeo*>(D + L.geo_items);
+    ga.pts = reinterpret_cast<const double*>(D + L.geo_pts);
+    ga.gl = reinterpret_cast<const double*>(D + L.gl);
+    ga.geo = reinterpret_cast<double*>(D + L.geo);
+    ga.n_items = (uint32_t)geo_i;
     ra.req_of = nullptr;
     ra.group_end = reinterpret_cast<const uint64_t*>(D + L.group_end);
     ra.out_end = reinterpret_cast<const uint64_t*>(D + L.out_end);
@@ -537,7 +602,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         int bpc = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc,
                                                           path_kernel_symbol(prec, b->ppt[cls], PathMode::Normal,
-                                                                             class_martingale(cls)),
+                                                                             class_martingale(cls), class_geo(cls)),
                                                           (int)block, lds) != hipSuccess ||
             bpc <= 0)
             bpc = std::max(1, 1024 / (int)block);
@@ -560,6 +625,12 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         err = "upload of the batch tables failed";
         return HEXO_ERR_HIP;
     }
+    // the exact geometric prices depend on the requests only: computed once, before any run
+    if (ga.n_items && launch_geo(ga, stream) != hipSuccess) {
+        batch_destroy(*b);
+        err = "launch of the geometric control-variate prices failed";
+        return HEXO_ERR_HIP;
+    }
     out = std::move(b);
     return HEXO_OK;
 }
@@ -574,7 +645,7 @@ static int enqueue_body(Batch& b, PathMode mode, bool timed, double* d_finals, s
         a.finals = d_finals;
         a.counts = mode == PathMode::Count ? b.d_counts : nullptr;
         HIP_TRY(launch_paths(class_precision(cls), b.ppt[cls], mode, b.grid[cls], b.block[cls], b.stream, a,
-                             class_martingale(cls)));
+                             class_martingale(cls), class_geo(cls)));
         ++b.path_launches;
     }
     if (timed) HIP_TRY(hipEventRecord(b.ev[1], b.stream));

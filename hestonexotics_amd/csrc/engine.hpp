@@ -37,6 +37,29 @@ struct HostSeg {
 bool build_schedule(const double* T, uint32_t n_exp, const uint32_t* n_strikes, uint32_t steps,
                     std::vector<HostSeg>& segs, uint64_t& total_steps);
 
+// ---- geometric-average control variate (geo_host.cpp) -------------------------------------
+constexpr uint32_t kGeoMaxPanels = 4096;   // 64-node panels per exact price before it reads NaN
+constexpr uint32_t kGeoMaxPoints = 65536;  // grid steps of one control-variate expiry
+struct HostGeoItem {
+    uint32_t pts_begin = 0;  // request-relative (t_j, v_j) pair index
+    uint32_t n_pts = 0;
+    uint32_t n_strikes = 0;
+    uint32_t opt = 0;  // request-relative first option of the block
+    uint32_t max_panels = 0;
+    double c = 0.0, h = 0.0;
+};
+struct GeoPlan {
+    std::vector<double> fin_W;  // per finalisation (schedule order): W of its Asian functional, 1 if unused
+    std::vector<double> pts;    // (t_j, v_j) pairs of every control-variate expiry
+    std::vector<HostGeoItem> items;
+};
+// Weights w[0..n] (and times t[0..n]) of chain `chain`'s Asian functional A = sum_j w_j X_{t_j}
+// as the path kernels accumulate it over the run schedule; returns n, or -1 if never finalised.
+int64_t functional_weights(const std::vector<HostSeg>& segs, uint32_t flags, uint32_t chain, std::vector<double>& t,
+                           std::vector<double>& w);
+bool plan_geo(const hexo_request* r, const std::vector<HostSeg>& segs, GeoPlan& g, std::string& why);
+void gauss_legendre64(double* x, double* w);
+
 std::string validate(const hexo_request* r);  // "" if valid, else the reason
 uint32_t n_payoffs(uint32_t payoffs);
 uint64_t n_opts_of(const hexo_request* r);
@@ -70,9 +93,12 @@ struct PinnedBuf {
 };
 
 // ---- a batch of requests resident on the device --------------------------------------------
-constexpr uint32_t kPathClasses = 3;
+// kernel classes: 0 = FP64 kernel; FP32 kernels 1 + (martingale-corrected step) + 2 (geometric
+// control variate's accumulation)
+constexpr uint32_t kPathClasses = 5;
 inline uint32_t class_precision(uint32_t cls) { return cls == 0 ? HEXO_FP64 : HEXO_FP32; }
-inline bool class_martingale(uint32_t cls) { return cls == 2; }
+inline bool class_martingale(uint32_t cls) { return cls == 2 || cls == 4; }
+inline bool class_geo(uint32_t cls) { return cls >= 3; }
 
 struct Batch {
     hexo_ctx* ctx = nullptr;
@@ -92,12 +118,11 @@ struct Batch {
     size_t upload_bytes = 0;
     double* h_prices = nullptr;
     double* h_ses = nullptr;
-    // kernel classes: 0 = FP64 kernel, 1 = FP32 kernel, 2 = FP32 kernel with the
-    // martingale-corrected step
     PathArgs pa[kPathClasses]{};
     uint32_t grid[kPathClasses]{}, block[kPathClasses]{}, ppt[kPathClasses]{};
     bool has_class[kPathClasses]{};
     ReduceArgs ra{};
+    GeoArgs ga{};  // exact geometric prices, computed once after the upload (n_items 0: none)
     uint64_t n_group_threads = 0, n_out_threads = 0;
     uint32_t* d_counters = nullptr;
     unsigned long long* d_counts = nullptr;

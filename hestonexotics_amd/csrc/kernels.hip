@@ -16,6 +16,8 @@
 // pairs; each tile's sums therefore depend only on its 64 paths, never on the launch shape.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device.hpp"
 #include "kernels.hpp"
 #include "layout.hpp"
@@ -24,7 +26,8 @@ namespace hexo {
 
 namespace {
 
-constexpr uint32_t kAsian = 1u;
+constexpr uint32_t kEuropean = 2u;
+constexpr uint32_t kAsianGeoCV = 4u;
 constexpr uint32_t kTrapezoid = 4u;
 constexpr uint32_t kMartingale = 8u;
 constexpr uint32_t kPrecFP64 = 0u;
@@ -132,13 +135,29 @@ struct TileSums<float> {  // compensated FP32 sums (HEXO_FP32_COMPENSATED)
     }
 };
 
-// payoff of one path value for strike K; `v` is the FP32 kernel's float2 or the FP64 kernel's double2
-__device__ __forceinline__ float payoff_of(float2 v, bool use_xt, float K, bool) {
-    return __builtin_fmaxf((use_xt ? v.y : v.x) - K, 0.0f);
+// Payoff kind of output block `pay` of a request: the pay-th set bit of its payoffs (outputs are
+// payoff-major in bit order: Asian, European, Asian with the geometric control variate).
+__device__ __forceinline__ uint32_t payoff_kind(uint32_t payoffs, uint32_t pay) {
+    uint32_t b = payoffs;
+    for (uint32_t i = 0; i < pay; ++i) b &= b - 1u;
+    return b & (~b + 1u);
 }
-__device__ __forceinline__ double payoff_of(double2 v, bool use_xt, double K, bool valid) {
-    const double x = (use_xt ? v.y : v.x) - K;
-    return (valid && x > 0.0) ? x : 0.0;
+
+// Payoff of one path for strike K and payoff kind. Path values: the FP32 kernel's float2 (A, X_T)
+// or float4 (A, X_T, G, -) in units of S0, paths beyond n_paths holding -inf; the FP64 kernel's
+// double4 (A, X_T, G, -) with an explicit validity flag. The control-variate payoff is the
+// arithmetic minus the geometric call payoff.
+__device__ __forceinline__ float payoff_of(float2 v, uint32_t kind, float K, bool) {
+    return __builtin_fmaxf((kind == kEuropean ? v.y : v.x) - K, 0.0f);
+}
+__device__ __forceinline__ float payoff_of(float4 v, uint32_t kind, float K, bool) {
+    const float pa = __builtin_fmaxf((kind == kEuropean ? v.y : v.x) - K, 0.0f);
+    return kind == kAsianGeoCV ? pa - __builtin_fmaxf(v.z - K, 0.0f) : pa;
+}
+__device__ __forceinline__ double payoff_of(double4 v, uint32_t kind, double K, bool valid) {
+    const double pa = __builtin_fmax((kind == kEuropean ? v.y : v.x) - K, 0.0);
+    const double pay = kind == kAsianGeoCV ? pa - __builtin_fmax(v.z - K, 0.0) : pa;
+    return valid ? pay : 0.0;
 }
 
 template <int PPT, typename ValT, typename Acc>
@@ -147,9 +166,8 @@ __device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevRe
     using KT = decltype(ValT{}.x);
     const uint32_t nk = F.n_strikes;
     const uint32_t n_out = R.n_opts * R.n_pay;
-    const bool asian = (R.payoffs & kAsian) != 0;
     for (uint32_t pay = 0; pay < R.n_pay; ++pay) {
-        const bool use_xt = pay == 1u || !asian;  // wave-uniform
+        const uint32_t kind = payoff_kind(R.payoffs, pay);  // wave-uniform
         if (nk >= kWideStrikes) {
             for (uint32_t kb = 0; kb < nk; kb += kTile) {
                 const uint32_t kj = kb + (uint32_t)lane;
@@ -163,7 +181,7 @@ __device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevRe
                     if (tile >= R.n_tiles) break;
                     const uint64_t nv = R.n_paths - (uint64_t)tile * kTile;
                     TileSums<Acc> t;
-                    for (int p = 0; p < kTile; ++p) t.add((Acc)payoff_of(vals[j * kTile + p], use_xt, K, (uint64_t)p < nv));
+                    for (int p = 0; p < kTile; ++p) t.add((Acc)payoff_of(vals[j * kTile + p], kind, K, (uint64_t)p < nv));
                     if (active) t.store(a.partials, (size_t)R.partial_base + (size_t)tile * n_out + o);
                 }
             }
@@ -184,7 +202,7 @@ __device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevRe
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
                         const int p = g * 8 + i;
-                        t.add((Acc)payoff_of(vals[j * kTile + p], use_xt, K, (uint64_t)p < nv));
+                        t.add((Acc)payoff_of(vals[j * kTile + p], kind, K, (uint64_t)p < nv));
                     }
                     t.tree8();
                     if (g == 0 && active) t.store(a.partials, (size_t)R.partial_base + (size_t)tile * n_out + o);
@@ -197,7 +215,7 @@ __device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevRe
 template <int PPT, typename ValT>
 __device__ __forceinline__ void reduce_tiles(const PathArgs& a, const DevReq& R, const DevFin& F, uint32_t tile0,
                                              int lane, const ValT* vals, bool) {
-    if (sizeof(ValT) == sizeof(float2) && R.precision == kPrecComp)
+    if (std::is_same<decltype(ValT{}.x), float>::value && R.precision == kPrecComp)
         reduce_tiles_impl<PPT, ValT, float>(a, R, F, tile0, lane, vals);
     else
         reduce_tiles_impl<PPT, ValT, double>(a, R, F, tile0, lane, vals);
@@ -277,10 +295,12 @@ struct StepCount {
 // branch; then the central log-price step. Y is log2(X/S0); X = 2^Y; P accumulates X for the
 // Asian sum. Z_V (quadratic lanes) and Z_X come from AS241 with the tails gathered per wave.
 // MART: Andersen's martingale-corrected step (the DevSeg form), falling back per path and step to
-// the plain step where E[exp(A V')] does not exist.
-template <int PPT, bool COUNT, bool MART>
+// the plain step where E[exp(A V')] does not exist. GEO: also keeps Y before the step (Yp) and the
+// run's sum of Y (PY), for the geometric control variate.
+template <int PPT, bool COUNT, bool MART, bool GEO>
 __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], float (&X)[PPT], float (&Xp)[PPT],
-                                             float (&P)[PPT], const uint32_t (&wv)[PPT], const uint32_t (&wx)[PPT],
+                                             float (&P)[PPT], float (&Yp)[PPT], float (&PY)[PPT],
+                                             const uint32_t (&wv)[PPT], const uint32_t (&wx)[PPT],
                                              const SegF& c, uint32_t* q, int lane, const bool (&valid)[PPT],
                                              StepCount& cnt) {
     float m[PPT], s2[PPT], m2[PPT];
@@ -383,10 +403,12 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
             inc = mok ? im : inc;
         }
         const float sq = __builtin_amdgcn_sqrtf(c.k3 * sum);
+        if constexpr (GEO) Yp[j] = Y[j];
         Y[j] = __builtin_fmaf(sq, z[2 * j + 1], Y[j] + inc);
         Xp[j] = X[j];
         X[j] = __builtin_amdgcn_exp2f(Y[j]);
         P[j] += X[j];
+        if constexpr (GEO) PY[j] += Y[j];
         V[j] = Vn;
     }
 }
@@ -405,12 +427,13 @@ __device__ __forceinline__ void philox_block(uint32_t blk, const uint32_t (&plo)
     }
 }
 
-template <int PPT, bool DEBUG, bool COUNT, bool MART>
+template <int PPT, bool DEBUG, bool COUNT, bool MART, bool GEO>
 __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32_t tile0, int lane, uint32_t* lds) {
     const DevReq& R = a.reqs[ri];
     const uint64_t n_paths = R.n_paths;
     const uint32_t key0 = R.seed_lo, key1 = R.seed_hi;
     float V[PPT], Y[PPT], X[PPT], Xp[PPT], P[PPT], acc[PPT], Xs[PPT];
+    float Yp[PPT], PY[PPT], accY[PPT], Ys[PPT];  // GEO: the same accumulation on Y = log2(X/S0)
     uint32_t plo[PPT], phi[PPT], sv[PPT], sx[PPT];
     bool valid[PPT];
     StepCount cnt;
@@ -430,6 +453,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
         P[j] = 0.0f;
         acc[j] = 0.0f;
         Xs[j] = 1.0f;
+        Yp[j] = PY[j] = accY[j] = Ys[j] = 0.0f;
         sv[j] = sx[j] = 0u;
     }
     uint32_t k = 0;  // global step index; step k uses words 2k, 2k+1 of the path's stream
@@ -440,28 +464,33 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
         const uint32_t n = S.n_steps;
         uint32_t i = 0;
         if ((k & 1u) && n > 0) {  // second half of a block opened by the previous run
-            qe_step_fp32<PPT, COUNT, MART>(V, Y, X, Xp, P, sv, sx, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, sv, sx, c, lds, lane, valid, cnt);
             ++i;
             ++k;
         }
         for (; i + 2 <= n; i += 2, k += 2) {
             uint32_t w0[PPT], w1[PPT], w2[PPT], w3[PPT];
             philox_block<PPT>(k >> 1, plo, phi, key0, key1, w0, w1, w2, w3);
-            qe_step_fp32<PPT, COUNT, MART>(V, Y, X, Xp, P, w0, w1, c, lds, lane, valid, cnt);
-            qe_step_fp32<PPT, COUNT, MART>(V, Y, X, Xp, P, w2, w3, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w0, w1, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w2, w3, c, lds, lane, valid, cnt);
         }
         if (i < n) {
             uint32_t w0[PPT], w1[PPT];
             philox_block<PPT>(k >> 1, plo, phi, key0, key1, w0, w1, sv, sx);
-            qe_step_fp32<PPT, COUNT, MART>(V, Y, X, Xp, P, w0, w1, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w0, w1, c, lds, lane, valid, cnt);
             ++k;
         }
         // Asian accumulation of the run's interior steps b..e-1:
         // delta * (X_{b-1}/2 + X_b + ... + X_{e-2} + X_{e-1}/2) = delta * (P - X_e + (Xs - Xp)/2)
 #pragma unroll
-        for (int j = 0; j < PPT; ++j)
+        for (int j = 0; j < PPT; ++j) {
             acc[j] = __builtin_fmaf(S.delta_f, (P[j] - X[j]) + 0.5f * (Xs[j] - Xp[j]), acc[j]);
-        float2* vals = reinterpret_cast<float2*>(lds);
+            if constexpr (GEO) accY[j] = __builtin_fmaf(S.delta_f, (PY[j] - Y[j]) + 0.5f * (Ys[j] - Yp[j]), accY[j]);
+        }
+        // per-path values at an expiry: (A, X_T), with GEO (A, X_T, G, -); the wave's LDS queue
+        // holds 4 words per path either way
+        using Val = typename std::conditional<GEO, float4, float2>::type;
+        Val* vals = reinterpret_cast<Val*>(lds);
         for (uint32_t f = 0; f < S.fin_count; ++f) {
             const DevFin& F = a.fins[S.fin_begin + f];
             const float w = F.w_f, invT = F.invT_f;
@@ -472,7 +501,18 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
                 const float A = (R.flags & kTrapezoid) ? (acc[j] + F.hwdt_f * (Xp[j] + xt)) * invT
                                                        : __builtin_fmaf(dX, w, acc[j]) * invT;
                 // paths beyond n_paths carry -inf: every payoff max(0, . - K) is then 0
-                vals[j * kTile + lane] = valid[j] ? make_float2(A, xt) : make_float2(-INFINITY, -INFINITY);
+                if constexpr (GEO) {
+                    // the same functional on Y, normalised by its weight sum W: G/S0 = W 2^(L/W)
+                    const float dY = Y[j] - Yp[j];
+                    const float L = (R.flags & kTrapezoid)
+                                        ? (accY[j] + F.hwdt_f * (Yp[j] + __builtin_fmaf(dY, w, Yp[j]))) * invT
+                                        : __builtin_fmaf(dY, w, accY[j]) * invT;
+                    const float G = F.gW_f * __builtin_amdgcn_exp2f(L * F.gInvW_f);
+                    vals[j * kTile + lane] =
+                        valid[j] ? make_float4(A, xt, G, 0.0f) : make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+                } else {
+                    vals[j * kTile + lane] = valid[j] ? make_float2(A, xt) : make_float2(-INFINITY, -INFINITY);
+                }
                 if constexpr (DEBUG) {
                     if (valid[j] && a.finals) {
                         const uint64_t pl = (uint64_t)(tile0 + (uint32_t)j) * kTile + (uint64_t)lane;
@@ -492,12 +532,17 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
             acc[j] = __builtin_fmaf(hn, X[j] + Xp[j], acc[j]);
             Xs[j] = X[j];
             P[j] = 0.0f;
+            if constexpr (GEO) {
+                accY[j] = __builtin_fmaf(hn, Y[j] + Yp[j], accY[j]);
+                Ys[j] = Y[j];
+                PY[j] = 0.0f;
+            }
         }
     }
     if constexpr (COUNT) count_flush(a.counts, cnt.quad, cnt.exp, cnt.central, cnt.tail, lane);
 }
 
-template <int PPT, bool DEBUG, bool COUNT, bool MART>
+template <int PPT, bool DEBUG, bool COUNT, bool MART, bool GEO>
 __global__ __launch_bounds__(256) void k_path_fp32(PathArgs a) {
     extern __shared__ uint32_t lds_u32[];
     const int lane = (int)(threadIdx.x & 63u);
@@ -508,7 +553,7 @@ __global__ __launch_bounds__(256) void k_path_fp32(PathArgs a) {
         const uint32_t sp = uniform(find_sorted(a.item_end, a.n_sorted, item));
         const uint32_t first = sp ? a.item_end[sp - 1] : 0u;
         const uint32_t ri = uniform(a.order[sp]);
-        path_fp32<PPT, DEBUG, COUNT, MART>(a, ri, uniform((item - first) * (uint32_t)PPT), lane, lds);
+        path_fp32<PPT, DEBUG, COUNT, MART, GEO>(a, ri, uniform((item - first) * (uint32_t)PPT), lane, lds);
     }
 }
 // ------------------------------------------------------------------------------------------
@@ -561,8 +606,10 @@ __device__ __forceinline__ bool qe_step_fp64(const DevReq& R, double delta, doub
 __device__ __forceinline__ bool central_f64(uint32_t k) { return __builtin_fabs(uniform_f64(k) - 0.5) <= 0.425; }
 
 template <bool DEBUG, bool COUNT>
-__device__ __forceinline__ void path_fp64(const PathArgs& a, uint32_t ri, uint32_t tile, int lane, double2* vals) {
+__device__ __forceinline__ void path_fp64(const PathArgs& a, uint32_t ri, uint32_t tile, int lane, double4* vals) {
     const DevReq& R = a.reqs[ri];
+    const bool geo = (R.payoffs & kAsianGeoCV) != 0;  // wave-uniform
+    double accY = 0.0, yp = 0.0;  // the Asian accumulation on y = ln(X/S0); y before the last step
     const uint64_t pl = (uint64_t)tile * kTile + (uint64_t)lane;
     const bool valid = pl < R.n_paths;
     const uint64_t gp = R.path_offset + pl;
@@ -586,6 +633,7 @@ __device__ __forceinline__ void path_fp64(const PathArgs& a, uint32_t ri, uint32
                 wv = wb0;
                 wx = wb1;
             }
+            yp = logX - R.log_S0;
             const bool quad = qe_step_fp64(R, delta, V, logX, wv, wx);
             Xp = X;
             X = ::exp(logX);
@@ -598,14 +646,25 @@ __device__ __forceinline__ void path_fp64(const PathArgs& a, uint32_t ri, uint32
                     ct += (quad && !cv ? 1u : 0u) + (cx ? 0u : 1u);
                 }
             }
-            if (i + 1u < n) acc += delta * 0.5 * (X + Xp);  // AAsianCallNonAdaptive::accumulate_value
+            if (i + 1u < n) {
+                acc += delta * 0.5 * (X + Xp);  // AAsianCallNonAdaptive::accumulate_value
+                if (geo) accY += delta * 0.5 * ((logX - R.log_S0) + yp);
+            }
         }
         for (uint32_t f = 0; f < S.fin_count; ++f) {
             const DevFin& F = a.fins[S.fin_begin + f];
             const double xt = Xp + (X - Xp) * (F.T - F.tp) / F.dt;
             const double A = (R.flags & kTrapezoid) ? (acc + (F.T - F.tp) * 0.5 * (Xp + xt)) / F.T
                                                     : (acc + (X - Xp) * (F.T - F.tp) / F.dt) / F.T;
-            vals[lane] = make_double2(A, xt);
+            double G = 0.0;
+            if (geo) {
+                // the same functional on y, normalised by its weight sum W: G = W S0 exp(L / W)
+                const double y = logX - R.log_S0, wf = (F.T - F.tp) / F.dt;
+                const double L = (R.flags & kTrapezoid) ? (accY + (F.T - F.tp) * 0.5 * (yp + (yp + (y - yp) * wf))) / F.T
+                                                        : (accY + (y - yp) * wf) / F.T;
+                G = F.gW * R.S0 * ::exp(L / F.gW);
+            }
+            vals[lane] = make_double4(A, xt, G, 0.0);
             if constexpr (DEBUG) {
                 if (valid && a.finals) {
                     double* out = a.finals + (pl * R.n_exp + F.chain) * 2;
@@ -618,15 +677,16 @@ __device__ __forceinline__ void path_fp64(const PathArgs& a, uint32_t ri, uint32
             wave_sync_lds();
         }
         acc += S.dnext * 0.5 * (X + Xp);
+        if (geo) accY += S.dnext * 0.5 * ((logX - R.log_S0) + yp);
     }
     if constexpr (COUNT) count_flush(a.counts, cq, ce, cc, ct, lane);
 }
 
 template <bool DEBUG, bool COUNT>
 __global__ __launch_bounds__(256) void k_path_fp64(PathArgs a) {
-    extern __shared__ double2 lds_d2[];
+    extern __shared__ double4 lds_d4[];
     const int lane = (int)(threadIdx.x & 63u);
-    double2* vals = lds_d2 + (threadIdx.x >> 6) * kTile;
+    double4* vals = lds_d4 + (threadIdx.x >> 6) * kTile;
     for (;;) {
         const uint32_t item = next_item(a.counter, lane);
         if (item >= a.total_items) break;
@@ -738,7 +798,9 @@ __global__ __launch_bounds__(256) void k_reduce_final(ReduceArgs a, uint64_t tot
     const double mean = s / n;
     double var = n > 1.0 ? (ss - n * mean * mean) / (n - 1.0) : 0.0;
     var = var > 0.0 ? var : 0.0;
-    a.prices[R.out_base + o] = scale * disc * mean;
+    // control variate: mean of (arithmetic - geometric payoff) plus the exact geometric price
+    const bool cv = a.geo && payoff_kind(R.payoffs, o / R.n_opts) == kAsianGeoCV;
+    a.prices[R.out_base + o] = disc * (scale * mean + (cv ? a.geo[R.out_base + o] : 0.0));
     if (a.std_errs) a.std_errs[R.out_base + o] = scale * disc * ::sqrt(var / n);
 }
 
@@ -796,7 +858,7 @@ inline dim3 grid_for(uint64_t n, uint32_t block) { return dim3((uint32_t)((n + b
 // ------------------------------------------------------------------------------------------
 // Launch wrappers (host)
 // ------------------------------------------------------------------------------------------
-const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode, bool mart) {
+const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode, bool mart, bool geo) {
     if (precision == kPrecFP64) {
         switch (mode) {
             case PathMode::Normal: return reinterpret_cast<const void*>(&k_path_fp64<false, false>);
@@ -805,19 +867,25 @@ const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode, 
         }
         return nullptr;
     }
-#define HEXO_PPT_MODES(P, M)                                                                         \
-    switch (mode) {                                                                                  \
-        case PathMode::Normal: return reinterpret_cast<const void*>(&k_path_fp32<P, false, false, M>); \
-        case PathMode::Debug: return reinterpret_cast<const void*>(&k_path_fp32<P, true, false, M>);   \
-        case PathMode::Count: return reinterpret_cast<const void*>(&k_path_fp32<P, false, true, M>);   \
-    }                                                                                                \
+#define HEXO_PPT_MODES(P, M, G)                                                                          \
+    switch (mode) {                                                                                      \
+        case PathMode::Normal: return reinterpret_cast<const void*>(&k_path_fp32<P, false, false, M, G>); \
+        case PathMode::Debug: return reinterpret_cast<const void*>(&k_path_fp32<P, true, false, M, G>);   \
+        case PathMode::Count: return reinterpret_cast<const void*>(&k_path_fp32<P, false, true, M, G>);   \
+    }                                                                                                    \
     return nullptr;
-#define HEXO_PPT_CASE(P)          \
-    case P:                       \
-        if (mart) {               \
-            HEXO_PPT_MODES(P, true)  \
-        }                         \
-        HEXO_PPT_MODES(P, false)
+#define HEXO_PPT_CASE(P)                    \
+    case P:                                 \
+        if (mart && geo) {                  \
+            HEXO_PPT_MODES(P, true, true)   \
+        }                                   \
+        if (mart) {                         \
+            HEXO_PPT_MODES(P, true, false)  \
+        }                                   \
+        if (geo) {                          \
+            HEXO_PPT_MODES(P, false, true)  \
+        }                                   \
+        HEXO_PPT_MODES(P, false, false)
     switch (ppt) {
         HEXO_PPT_CASE(1)
         HEXO_PPT_CASE(2)
@@ -830,12 +898,13 @@ const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode, 
 
 size_t path_lds_bytes(uint32_t precision, uint32_t ppt, uint32_t block) {
     const size_t waves = block / kTile;
-    // FP32: per wave the tail-normal queue (2 * 2 * ppt * 64 words), reused by the (A, X_T) transpose
-    return precision == kPrecFP64 ? waves * kTile * sizeof(double2) : waves * 4 * ppt * kTile * sizeof(uint32_t);
+    // FP32: per wave the tail-normal queue (2 * 2 * ppt * 64 words), reused by the transpose of the
+    // per-path values at expiries (up to 4 words per path); FP64: 64 double4 per wave
+    return precision == kPrecFP64 ? waves * kTile * sizeof(double4) : waves * 4 * ppt * kTile * sizeof(uint32_t);
 }
 
 hipError_t launch_paths(uint32_t precision, uint32_t ppt, PathMode mode, uint32_t grid, uint32_t block,
-                        hipStream_t st, const PathArgs& a, bool mart) {
+                        hipStream_t st, const PathArgs& a, bool mart, bool geo) {
     const size_t lds = path_lds_bytes(precision, ppt, block);
     if (precision == kPrecFP64) {
         switch (mode) {
@@ -845,17 +914,21 @@ hipError_t launch_paths(uint32_t precision, uint32_t ppt, PathMode mode, uint32_
         }
         return hipGetLastError();
     }
-#define HEXO_LAUNCH_M(P, M)                                                                      \
-    switch (mode) {                                                                              \
-        case PathMode::Normal: k_path_fp32<P, false, false, M><<<grid, block, lds, st>>>(a); break; \
-        case PathMode::Debug: k_path_fp32<P, true, false, M><<<grid, block, lds, st>>>(a); break;   \
-        case PathMode::Count: k_path_fp32<P, false, true, M><<<grid, block, lds, st>>>(a); break;   \
+#define HEXO_LAUNCH_M(P, M, G)                                                                      \
+    switch (mode) {                                                                                 \
+        case PathMode::Normal: k_path_fp32<P, false, false, M, G><<<grid, block, lds, st>>>(a); break; \
+        case PathMode::Debug: k_path_fp32<P, true, false, M, G><<<grid, block, lds, st>>>(a); break;   \
+        case PathMode::Count: k_path_fp32<P, false, true, M, G><<<grid, block, lds, st>>>(a); break;   \
     }
-#define HEXO_LAUNCH(P)            \
-    if (mart) {                   \
-        HEXO_LAUNCH_M(P, true)    \
-    } else {                      \
-        HEXO_LAUNCH_M(P, false)   \
+#define HEXO_LAUNCH(P)                  \
+    if (mart && geo) {                  \
+        HEXO_LAUNCH_M(P, true, true)    \
+    } else if (mart) {                  \
+        HEXO_LAUNCH_M(P, true, false)   \
+    } else if (geo) {                   \
+        HEXO_LAUNCH_M(P, false, true)   \
+    } else {                            \
+        HEXO_LAUNCH_M(P, false, false)  \
     }
     switch (ppt) {
         case 1: HEXO_LAUNCH(1) break;

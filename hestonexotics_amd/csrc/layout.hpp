@@ -51,13 +51,14 @@ struct DevSeg {
 // Finalisation of one expiry (AAsianCallNonAdaptive / EuropeanCallNonAdaptive
 // accumulate_final_value, src/inc/AsianContract.h:29-34, src/inc/VanillaContract.h:28-31).
 struct DevFin {
-    uint32_t chain, opt_begin, n_strikes, pad0;
+    uint32_t chain, opt_begin, n_strikes;
+    float gInvW_f;  // geometric control variate: 1 / W
     float w_f;     // (T - t_prev) / dt
     float invT_f;  // 1 / T
     float hwdt_f;  // (T - t_prev) / 2, corrected-trapezoid extension
-    float pad1;
+    float gW_f;    // geometric control variate: W, the sum of the Asian functional's weights
     double T, tp, dt;  // FP64 kernel: reference-form interpolation (X - Xp)(T - tp)/dt
-    double pad2;
+    double gW;         // FP64 kernel: W
 };
 
 struct DevOpt {
@@ -94,9 +95,36 @@ struct PathArgs {
     unsigned long long* counts;     // optional branch counters [4]
 };
 
+// Geometric-average control variate (payoff HEXO_ASIAN_GEO_CV). The Asian functional of an
+// expiry, A = sum_j w_j X_{t_j} over the grid points, has weight sum W; its geometric counterpart
+// is G = W S0 exp(sum_j v_j ln(X_{t_j}/S0)) with v = w / W, whose exact price under the Heston
+// model is computed once per batch: one wavefront per (request, expiry, block of <= 64 strikes).
+struct DevGeo {
+    uint32_t req;         // request index
+    uint32_t pts_begin;   // first point (t_0 = 0) of the expiry's (t_j, v_j) list
+    uint32_t n_pts;       // intervals: points 0..n_pts
+    uint32_t n_strikes;   // strikes of this block (<= 64)
+    uint32_t opt;         // first option of the block (batch-global DevOpt index)
+    uint32_t max_panels;  // bound on 64-node panels (the price is NaN if not converged by then)
+    uint64_t out;         // geo[] index of the block's first control-variate output
+    double c;             // W S0
+    double h;             // width in u of one panel
+};
+
+struct GeoArgs {
+    const DevReq* reqs;
+    const DevOpt* opts;
+    const DevGeo* items;
+    const double* pts;  // (t_j, v_j) pairs
+    const double* gl;   // 64 Gauss-Legendre nodes on [-1, 1], then their 64 weights
+    double* geo;        // exact geometric prices (undiscounted, absolute units), per output
+    uint32_t n_items;
+};
+
 struct ReduceArgs {
     const DevReq* reqs;
     const DevOpt* opts;
+    const double* geo;  // exact geometric prices added to the control-variate outputs (or null)
     const uint32_t* req_of;       // request index for sorted reduction slices
     const uint64_t* group_end;    // prefix over requests of n_groups * n_out (pass A threads)
     const uint64_t* out_end;      // prefix over requests of n_out (pass B threads)

@@ -14,6 +14,7 @@
 // characteristic-function expression evaluated in forward-mode dual arithmetic carrying all five
 // parameter directions, instead of hand-derived partials (src/HDistribution.cpp:47-84).
 #include "swift.hpp"
+#include "cplx_device.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -21,52 +22,6 @@ namespace hexo {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
-
-__device__ __forceinline__ Cplx cx(double re, double im = 0.0) { return Cplx{re, im}; }
-__device__ __forceinline__ Cplx operator+(Cplx a, Cplx b) { return Cplx{a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ Cplx operator-(Cplx a, Cplx b) { return Cplx{a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ Cplx operator*(Cplx a, Cplx b) {
-    return Cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-__device__ __forceinline__ Cplx operator*(double s, Cplx a) { return Cplx{s * a.re, s * a.im}; }
-__device__ __forceinline__ Cplx conj(Cplx a) { return Cplx{a.re, -a.im}; }
-// Smith's division: no overflow or underflow from forming |b|^2
-__device__ __forceinline__ Cplx operator/(Cplx a, Cplx b) {
-    if (fabs(b.re) >= fabs(b.im)) {
-        const double t = b.im / b.re, den = b.re + b.im * t;
-        return Cplx{(a.re + a.im * t) / den, (a.im - a.re * t) / den};
-    }
-    const double t = b.re / b.im, den = b.re * t + b.im;
-    return Cplx{(a.re * t + a.im) / den, (a.im * t - a.re) / den};
-}
-__device__ __forceinline__ Cplx cexp(Cplx a) {
-    double s, c;
-    sincos(a.im, &s, &c);
-    const double e = exp(a.re);
-    return Cplx{e * c, e * s};
-}
-__device__ __forceinline__ Cplx clog(Cplx a) { return Cplx{log(hypot(a.re, a.im)), atan2(a.im, a.re)}; }
-// principal square root (branch cut on the negative real axis, sign of the imaginary part kept)
-__device__ __forceinline__ Cplx csqrt(Cplx a) {
-    const double r = hypot(a.re, a.im);
-    if (r == 0.0) return Cplx{0.0, a.im};
-    if (a.re >= 0.0) {
-        const double t = sqrt(0.5 * (r + a.re));
-        return Cplx{t, a.im / (2.0 * t)};
-    }
-    const double t = sqrt(0.5 * (r - a.re));
-    return Cplx{fabs(a.im) / (2.0 * t), copysign(t, a.im)};
-}
-__device__ __forceinline__ Cplx csinh(Cplx a) {
-    double s, c;
-    sincos(a.im, &s, &c);
-    return Cplx{sinh(a.re) * c, cosh(a.re) * s};
-}
-__device__ __forceinline__ Cplx ccosh(Cplx a) {
-    double s, c;
-    sincos(a.im, &s, &c);
-    return Cplx{cosh(a.re) * c, sinh(a.re) * s};
-}
 
 // Plain complex versions of the elementary functions, so that the characteristic-function
 // expression below instantiates for Cplx (values) and for Dual5 (values and partials).

@@ -5,11 +5,9 @@
 
 #include <cstdint>
 
-namespace hexo {
+#include "cplx.hpp"
 
-struct Cplx {
-    double re, im;
-};
+namespace hexo {
 
 // Chains with J up to this size run both transforms as radix-2 FFTs of length 2J in one
 // workgroup's LDS (16 B x 2J = 128 KiB); larger ones use the direct-sum kernels.

@@ -42,6 +42,13 @@ typedef enum hexo_status {
 /* payoffs: which option policies are priced from the same paths */
 #define HEXO_ASIAN 1u    /* AAsianCallNonAdaptive   (src/inc/AsianContract.h:14-48) */
 #define HEXO_EUROPEAN 2u /* EuropeanCallNonAdaptive (src/inc/VanillaContract.h:14-40) */
+/* AAsianCallNonAdaptive priced with the geometric-average control variate the reference suggests
+ * (src/inc/HSimulation.h:48-60): per path max(A - K, 0) - max(G - K, 0), where G applies the same
+ * Asian functional to ln(X/S0) (normalised by its weight sum W: G = W S0 exp(sum_j w_j ln(X_j/S0) / W)),
+ * plus the exact price of the G call under the Heston model. Same expectation as the plain Asian
+ * estimator up to the scheme's discretisation of G; far smaller standard error. Needs every
+ * expiry's weight sum W >= 0.25 (at least 2 steps). Outputs are payoff-major in bit order. */
+#define HEXO_ASIAN_GEO_CV 4u
 
 /* flags: documented extensions; 0 reproduces the reference exactly */
 #define HEXO_DRIFT 1u     /* add r*delta to the log-price step (the reference has no drift term) */
@@ -81,7 +88,7 @@ typedef struct hexo_request {
     const double* expiries;
     const uint32_t* n_strikes;
     const double* strikes;
-    uint32_t payoffs;   /* HEXO_ASIAN | HEXO_EUROPEAN */
+    uint32_t payoffs;   /* HEXO_ASIAN | HEXO_EUROPEAN | HEXO_ASIAN_GEO_CV */
     uint32_t flags;     /* HEXO_DRIFT | HEXO_DISCOUNT | HEXO_TRAPEZOID | HEXO_MARTINGALE */
     uint32_t precision; /* HEXO_FP64 | HEXO_FP32 | HEXO_FP32_COMPENSATED */
     uint32_t reserved;  /* must be 0 */
@@ -116,13 +123,18 @@ const char* hexo_version(void);
  * model's domain; returns HEXO_OK or HEXO_ERR_INVALID. */
 int hexo_validate_request(const hexo_request* req);
 /* Number of outputs of a request: n_payoffs * sum(n_strikes). Outputs are payoff-major
- * (Asian block first), options in chain-major order. 0 if the request is invalid. */
+ * (Asian, European, control-variate Asian blocks, those requested), options in chain-major order. 0 if the request is invalid. */
 uint64_t hexo_request_n_outputs(const hexo_request* req);
 /* The reference's time grid (src/HSimulation.tpp:30-45 with src/inc/AsianContract.h:29-38),
  * replayed in double: fin_step[c] = 1-based step at which expiry c is finalised,
  * fin_w[c] = (T_c - t_prev) / dt_c; *total_steps = steps simulated per path. */
 int hexo_time_grid(const double* expiries, uint32_t n_expiries, const uint32_t* n_strikes, uint32_t steps,
                    uint64_t* fin_step, double* fin_w, uint64_t* total_steps);
+/* Weights of expiry `chain`'s Asian average as a linear functional of the path on the time grid,
+ * A = sum_j w[j] X(t[j]) for j = 0..n (t[0] = 0, X(0) = S0): the accumulation of
+ * AAsianCallNonAdaptive (src/inc/AsianContract.h:25-34) over the grid of src/HSimulation.tpp:30-45,
+ * as the path kernels execute it. Returns n (cap >= n + 1 entries needed) or HEXO_ERR_INVALID. */
+int64_t hexo_asian_weights(const hexo_request* req, uint32_t chain, double* t, double* w, uint64_t cap);
 
 /* ---- context ---- */
 int hexo_ctx_create(int device, hexo_ctx** out);
@@ -150,6 +162,13 @@ int hexo_batch_stats_get(hexo_batch* b, hexo_batch_stats* st);
  * counts[2] = AS241 central-region normals, counts[3] = AS241 tail normals. */
 int hexo_batch_branch_counts(hexo_batch* b, uint64_t counts[4]);
 int hexo_batch_destroy(hexo_batch* b);
+
+/* Exact price, under the Heston model, of the geometric counterpart of every option's Asian
+ * average (the control variate of HEXO_ASIAN_GEO_CV): E[max(G - K, 0)] per option, chain-major,
+ * times exp(-r T) with HEXO_DISCOUNT; FP64 on the GPU (affine characteristic-function recursion
+ * over the grid, Lewis' Fourier inversion). n_paths and the payoffs of req are ignored. No
+ * counterpart in the reference, which only suggests the control variate. */
+int hexo_geo_asian_price(hexo_ctx* ctx, const hexo_request* req, double* prices);
 
 /* ---- pathwise and component entry points (parity tests) ----
  * hexo_paths: per-path (Asian average A_c, interpolated X_T,c) at every expiry, for paths

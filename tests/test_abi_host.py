@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "hexo.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|uint64_t|const char\*)\s+(hexo_[a-z0-9_]+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|uint64_t|const char\*)\s+(hexo_[a-z0-9_]+)\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -57,6 +57,8 @@ MUTATIONS = {
     "flags": lambda r: setattr(r, "flags", 64),
     "precision": lambda r: setattr(r, "precision", 7),
     "path_offset_unaligned": lambda r: setattr(r, "path_offset", 100),
+    # the control variate needs an Asian average with weight sum >= 1/4 (one step has 0)
+    "control_variate_one_step": lambda r: (setattr(r, "payoffs", H.ASIAN_GEO_CV), setattr(r, "steps", 1)),
 }
 
 
@@ -88,6 +90,8 @@ def test_n_outputs():
     assert lib.hexo_request_n_outputs(C.byref(r.to_c())) == 8
     r.payoffs = H.EUROPEAN
     assert lib.hexo_request_n_outputs(C.byref(r.to_c())) == 4
+    r.payoffs = H.ASIAN | H.EUROPEAN | H.ASIAN_GEO_CV
+    assert lib.hexo_request_n_outputs(C.byref(r.to_c())) == 12
     r.steps = 0
     assert lib.hexo_request_n_outputs(C.byref(r.to_c())) == 0
 
@@ -137,3 +141,24 @@ def test_swift_grid_matches_oracle_and_rejects_bad_chains():
                  [5, 16, 5.65, -1.0, 1.0, -10, 10, 128]):   # 2^m inconsistent
         with pytest.raises(H.HexoError):
             H.swift_grid(H.SwiftChain(**ok, grid=grid))
+
+
+@pytest.mark.parametrize("T,strikes,N,flags", [
+    ([1.0], [[100.0]], 12, 0),                                     # last step on T
+    ([1.0], [[100.0]], 252, 0),                                    # one extra step
+    ([0.5, 1.0, 2.0], [[100.0], [], [90.0, 110.0]], 26, 0),        # step-width changes, an empty chain
+    ([0.3, 0.31, 0.7, 1.2], [[90, 100], [], [100, 105, 120], [100]], 7, H.TRAPEZOID),
+    ([0.1], [[100.0]], 2, H.TRAPEZOID),
+])
+def test_asian_weights_match_oracle(T, strikes, N, flags):
+    """The Asian functional's weights as the path kernels accumulate them (from the run schedule)
+    equal the oracle's, taken from the reference's own time loop: the control variate applies
+    them to ln X, and its exact price uses them."""
+    from oracle import oracle as O
+    req = H.Request(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0, expiries=T, strikes=strikes,
+                    steps=N, n_paths=64, flags=flags)
+    for c in range(len(T)):
+        t, w = H.asian_weights(req, c)
+        to, wo = O.asian_weights(T, [len(k) for k in strikes], N, c, flags)
+        np.testing.assert_array_equal(t, to)
+        np.testing.assert_allclose(w, wo, rtol=0, atol=1e-15)

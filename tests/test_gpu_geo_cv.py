@@ -1,0 +1,152 @@
+"""GPU: the geometric-average control variate of the Asian call (payoff ASIAN_GEO_CV).
+
+(1) The exact geometric prices computed on the GPU (hexo_geo_asian_price, the kernel every
+    control-variate batch runs once) equal the oracle's to 1e-9 S0: short and long maturities,
+    several expiries with an empty chain, the corrected trapezoid, the stiff and the peaked
+    (slowly decaying characteristic function) cases, positive correlation, more than 64 strikes,
+    a zero strike.
+(2) FP64 kernel: the Monte-Carlo part of the control-variate price, mean(max(A-K,0) - max(G-K,0)),
+    equals the oracle's on the same draws to 1e-12 relative; whole prices to 2e-9 S0.
+(3) FP32 against FP64 on identical draws within 1e-4; the standard error is a small fraction of the
+    plain Asian estimator's, and the two estimators agree within the plain standard error.
+(4) Mixed batches of all kernel classes return what each request returns alone.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import hestonexotics_amd as H
+from hestonexotics_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+BASE = dict(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def geo_requests():
+    return {
+        "c1_like": H.Request(**BASE, expiries=[1.0], strikes=[[80.0, 100.0, 120.0]], steps=252, n_paths=2000,
+                             seed=42, r=0.02, flags=H.DRIFT | H.DISCOUNT),
+        "multi_expiry": H.Request(**BASE, expiries=[0.3, 0.31, 0.7, 1.2],
+                                  strikes=[[95, 100], [], [100, 105, 120], [0.0, 100]], steps=7, n_paths=777,
+                                  seed=11, r=0.02, flags=H.DRIFT | H.TRAPEZOID),
+        "stiff_c5": H.Request(v0=0.04, theta=0.04, rho=-0.9, kappa=0.5, sigma=0.9, S0=100.0, expiries=[1.0],
+                              strikes=[[90.0, 100.0, 110.0]], steps=128, n_paths=640, seed=5, r=0.02,
+                              flags=H.DRIFT),
+        "peaked": H.Request(v0=0.01, theta=0.01, rho=-0.5, kappa=0.5, sigma=1.0, S0=100.0, expiries=[0.1],
+                            strikes=[[97.0, 100.0, 103.0]], steps=25, n_paths=500, seed=8),
+        "positive_rho": H.Request(v0=0.04, theta=0.09, rho=0.4, kappa=1.5, sigma=0.4, S0=100.0, expiries=[2.0],
+                                  strikes=[[90.0, 110.0]], steps=104, n_paths=500, seed=3, r=0.03,
+                                  flags=H.DRIFT | H.DISCOUNT),
+        "wide_strikes": H.Request(**BASE, expiries=[0.5], strikes=[list(np.linspace(60.0, 140.0, 70))], steps=26,
+                                  n_paths=300, seed=4),
+    }
+
+
+def cv(req, precision, payoffs=H.ASIAN_GEO_CV, flags=0):
+    return dataclasses.replace(req, precision=precision, payoffs=payoffs, flags=req.flags | flags)
+
+
+def oracle_price(oracle, req, kind="strict"):
+    return oracle.price(req.params, req.S0, req.expiries, req.n_strikes(), req.flat_strikes(), steps=req.steps,
+                        n_paths=req.n_paths, seed=req.seed, r=req.r, flags=req.flags, payoffs=req.payoffs,
+                        path_offset=req.path_offset, kind=kind)
+
+
+def discounts(req):
+    T = np.repeat(np.asarray(req.expiries, np.float64), req.n_strikes())
+    return np.exp(-req.r * T) if req.flags & H.DISCOUNT else np.ones_like(T)
+
+
+@pytest.mark.parametrize("name", sorted(geo_requests()))
+def test_exact_geometric_price_matches_oracle(ctx, oracle, name):
+    req = geo_requests()[name]
+    say("gpu geo exact", name)
+    g = ctx.geo_asian_price(req)
+    o = oracle.geo_asian_price(req.params, req.S0, req.expiries, req.n_strikes(), req.flat_strikes(),
+                               steps=req.steps, r=req.r if req.flags & H.DRIFT else 0.0, flags=req.flags)
+    o = o * discounts(req)
+    say(f"  max |gpu - oracle| / S0 = {np.abs(g - o).max() / req.S0:.2e}")
+    assert np.all(np.isfinite(g))
+    np.testing.assert_allclose(g, o, rtol=0, atol=1e-9 * req.S0)
+
+
+@pytest.mark.parametrize("name", sorted(geo_requests()))
+def test_fp64_control_variate_prices_equal_oracle(ctx, oracle, name):
+    req = cv(geo_requests()[name], H.FP64, payoffs=H.ASIAN | H.ASIAN_GEO_CV)
+    say("gpu geo cv fp64 prices", name)
+    p, s = ctx.price(req)
+    o = oracle_price(oracle, req)
+    n = req.n_opts
+    np.testing.assert_allclose(p[:n], o["prices"][:n], rtol=1e-12, atol=1e-12)   # plain Asian block
+    # the Monte-Carlo part of the control-variate block
+    exact_g = ctx.geo_asian_price(req)
+    exact_o = oracle.geo_asian_price(req.params, req.S0, req.expiries, req.n_strikes(), req.flat_strikes(),
+                                     steps=req.steps, r=req.r if req.flags & H.DRIFT else 0.0,
+                                     flags=req.flags) * discounts(req)
+    mc_g, mc_o = p[n:] - exact_g, o["prices"][n:] - exact_o
+    # a - g cancels most of a: the tolerance is absolute, 1e-11 S0, or twice the spread of two
+    # valid FP64 evaluations of the oracle (with and without FMA contraction)
+    o_fma = oracle_price(oracle, req, kind="fma")
+    spread = np.abs(o_fma["prices"][n:] - o["prices"][n:]).max()
+    tol = max(1e-11 * req.S0, 2.0 * spread)
+    say(f"  MC part max |gpu - oracle| {np.abs(mc_g - mc_o).max():.2e} (tol {tol:.1e})")
+    np.testing.assert_allclose(mc_g, mc_o, rtol=0, atol=tol)
+    np.testing.assert_allclose(s[n:], o["std_errs"][n:], rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(p[n:], o["prices"][n:], rtol=0, atol=2e-9 * req.S0)
+
+
+def test_fp32_control_variate_vs_fp64_and_variance_reduction(ctx):
+    say("gpu geo cv fp32 vs fp64, grid subset")
+    reqs = W.c2(n_paths=200_000, precision=H.FP64, maturities=[0.25, 1.0, 3.0], n_strikes=16)
+    for r in reqs:
+        r.payoffs = H.ASIAN | H.ASIAN_GEO_CV
+    r64 = ctx.price_batch(reqs)
+    for r in reqs:
+        r.precision = H.FP32
+    r32 = ctx.price_batch(reqs)
+    worst, ratios = 0.0, []
+    for q, (p64, s64), (p32, s32) in zip(reqs, r64, r32):
+        n = q.n_opts
+        big = p64[n:] > 1e-2
+        worst = max(worst, (np.abs(p32[n:][big] - p64[n:][big]) / p64[n:][big]).max())
+        atm = np.argmin(np.abs(q.flat_strikes() - 100.0))
+        ratios.append(s32[n + atm] / s32[atm])
+        # the two estimators of the same price agree within the plain one's standard error
+        assert np.all(np.abs(p32[n:] - p32[:n]) < 4 * s32[:n] + 2e-3), (q.expiries, p32[n:] - p32[:n])
+    say("  max rel err fp32 vs fp64:", worst, " ATM SE ratio cv/plain:", ratios)
+    assert worst < 1e-4
+    assert max(ratios) < 0.1
+
+
+def test_mixed_batch_equals_single(ctx):
+    say("gpu geo cv mixed batch == single")
+    base = geo_requests()["c1_like"]
+    variants = [
+        cv(base, H.FP64, H.ASIAN | H.ASIAN_GEO_CV),
+        cv(base, H.FP32, H.ASIAN),
+        cv(base, H.FP32, H.ASIAN_GEO_CV, H.MARTINGALE),
+        cv(base, H.FP32, H.ASIAN | H.EUROPEAN | H.ASIAN_GEO_CV),
+        cv(geo_requests()["multi_expiry"], H.FP32_COMPENSATED, H.EUROPEAN | H.ASIAN_GEO_CV),
+        cv(geo_requests()["stiff_c5"], H.FP32, H.ASIAN_GEO_CV, H.MARTINGALE),
+    ]
+    together = ctx.price_batch(variants)
+    for r, (p, s) in zip(variants, together):
+        p1, s1 = ctx.price(r)
+        assert np.all(np.isfinite(p))
+        np.testing.assert_array_equal(p, p1)
+        np.testing.assert_array_equal(s, s1)
+
+
+def test_control_variate_rejects_one_step(ctx):
+    req = cv(geo_requests()["c1_like"], H.FP32)
+    req.steps = 1
+    with pytest.raises(H.HexoError):
+        ctx.price(req)
+    with pytest.raises(H.HexoError):
+        ctx.geo_asian_price(req)
