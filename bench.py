@@ -243,6 +243,48 @@ def martingale_leg(ctx, args, plain_value: float, plain_kernel_ms: float):
             "terminal_mean_request": "v0 0.2, kappa 0.5, theta 0.04, sigma 0.9, rho -0.9, T 4, 4 steps, 2^24 paths"}
 
 
+def control_variate_leg(ctx, args, plain_value: float, plain_kernel_ms: float):
+    """The geometric-average control variate on the grid workload: Asian, European and
+    control-variate Asian from the same FP32 paths. Reports the path kernel's rate with the extra
+    accumulation, the one-off exact-price kernel, and the variance reduction: (plain SE / CV SE)^2
+    is the factor by which the control variate cuts the paths needed for a given accuracy."""
+    import hestonexotics_amd as H
+    from hestonexotics_amd import workloads as W
+
+    reqs = W.c2(n_paths=args.paths, precision=H.FP32)
+    for q in reqs:
+        q.payoffs = H.ASIAN | H.EUROPEAN | H.ASIAN_GEO_CV
+    nominal = W.path_steps(reqs)
+    steps = max(2, args.steps // 3)
+    b = ctx.batch(reqs)
+    wall, kms = time_batch(ctx, b, steps, 1)
+    pall, sall = b.results()
+    res = list(zip(b.per_request(pall), b.per_request(sall)))
+    b.close()
+    factors, atm = [], []
+    for q, (p, s) in zip(reqs, res):
+        d = q.split(s)
+        pr = q.split(p)
+        ok = (pr["asian"] > 1e-2) & (d["asian_geo_cv"] > 0)
+        factors.extend(list((d["asian"][ok] / d["asian_geo_cv"][ok]) ** 2))
+        k = int(np.argmin(np.abs(q.flat_strikes() - q.S0)))
+        atm.append(float((d["asian"][k] / d["asian_geo_cv"][k]) ** 2))
+    # the exact geometric prices: one wavefront per expiry, once per batch
+    t0 = time.perf_counter()
+    for q in reqs:
+        ctx.geo_asian_price(q)
+    geo_ms = 1e3 * (time.perf_counter() - t0) / len(reqs)
+    value = nominal * steps / wall
+    return {"workload": "C2 grid, Asian + European + control-variate Asian from the same FP32 paths",
+            "value": value, "kernel_ms": kms, "over_plain": value / plain_value,
+            "kernel_ms_plain": plain_kernel_ms,
+            "variance_reduction_median": float(np.median(factors)),
+            "variance_reduction_min": float(np.min(factors)),
+            "variance_reduction_atm_by_maturity": atm,
+            "exact_price_ms_per_request": geo_ms,
+            "effective_path_steps_per_s_median": value * float(np.median(factors))}
+
+
 def time_batch(ctx, batch, steps: int, warmup: int):
     for _ in range(warmup):
         batch.run_eager()
@@ -280,6 +322,7 @@ def main():
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c5-paths", type=int, default=2 ** 27, help="paths of the stiff precision-study request")
     ap.add_argument("--no-martingale", action="store_true")
+    ap.add_argument("--no-cv", action="store_true")
     ap.add_argument("--no-swift", action="store_true")
     ap.add_argument("--swift-params", type=int, default=64, help="parameter sets of the SWIFT calibration batch")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
@@ -289,8 +332,9 @@ def main():
                     help="profiling driver: --profile-passes eager FP32 passes of the workload, then a plain copy of "
                          "--copy-mib MiB (the known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--profile-passes", type=int, default=20)
-    ap.add_argument("--profile-workload", choices=("c2", "c3", "c5"), default="c2",
-                    help="workload of --profile: the grid (c2), the 512 uneven requests (c3), the stiff request (c5)")
+    ap.add_argument("--profile-workload", choices=("c2", "c3", "c5", "c2cv", "c2mart"), default="c2",
+                    help="workload of --profile: the grid (c2), the 512 uneven requests (c3), the stiff request (c5), "
+                         "the grid with the control-variate Asian (c2cv) or with the martingale correction (c2mart)")
     ap.add_argument("--profile-swift", action="store_true",
                     help="profiling driver for the SWIFT pricer: --profile-passes calls on the calibration batch")
     ap.add_argument("--copy-mib", type=int, default=512)
@@ -309,9 +353,18 @@ def main():
     info = ctx.device_info()
     log("device", info)
     if args.profile:
+        def with_(reqs, payoffs=0, flags=0):
+            for q in reqs:
+                q.payoffs |= payoffs
+                q.flags |= flags
+            return reqs
+
         prof_reqs = {"c2": lambda: W.c2(n_paths=args.paths, precision=H.FP32),
                      "c3": lambda: W.c3(precision=H.FP32),
-                     "c5": lambda: [W.c5(precision=H.FP32)]}[args.profile_workload]()
+                     "c5": lambda: [W.c5(precision=H.FP32)],
+                     "c2cv": lambda: with_(W.c2(n_paths=args.paths, precision=H.FP32), payoffs=H.ASIAN_GEO_CV),
+                     "c2mart": lambda: with_(W.c2(n_paths=args.paths, precision=H.FP32), flags=H.MARTINGALE),
+                     }[args.profile_workload]()
         bp = ctx.batch(prof_reqs)
         times = []
         for i in range(args.profile_passes):
@@ -417,6 +470,10 @@ def main():
     if not args.no_martingale:
         out["martingale"] = martingale_leg(ctx, args, value, kms32)
         log("martingale:", out["martingale"])
+
+    if not args.no_cv:
+        out["control_variate"] = control_variate_leg(ctx, args, value, kms32)
+        log("control variate:", out["control_variate"])
 
     if not args.no_c3:
         c3 = W.c3(precision=H.FP32)

@@ -216,14 +216,14 @@ struct ReqPlan {
 struct Layout {
     size_t reqs, segs, fins, opts, order[kPathClasses], item_end[kPathClasses], group_end, out_end;  // uploaded
     size_t upload_end;
-    size_t geo_items, geo_pts, gl;                                  // uploaded
-    size_t counters, counts, partials, groups, prices, ses, finals, geo;  // device only
+    size_t geo_items, geo_pts, gl, geo_wave_end;                                      // uploaded
+    size_t counters, counts, partials, groups, prices, ses, finals, geo, geo_panel, geo_pmax;  // device only
     size_t total;
 };
 
 Layout make_layout(uint32_t n_req, uint64_t n_seg, uint64_t n_fin, uint64_t n_opt, const uint32_t* n_cls,
                    uint64_t n_partials, uint64_t n_groups, uint64_t n_out, uint64_t n_finals, uint64_t n_geo_items,
-                   uint64_t n_geo_pts) {
+                   uint64_t n_geo_pts, uint64_t n_geo_waves) {
     Layout L{};
     size_t o = 0;
     auto take = [&](size_t bytes) {
@@ -244,6 +244,7 @@ Layout make_layout(uint32_t n_req, uint64_t n_seg, uint64_t n_fin, uint64_t n_op
     L.geo_items = take(sizeof(DevGeo) * std::max<uint64_t>(n_geo_items, 1));
     L.geo_pts = take(2 * sizeof(double) * std::max<uint64_t>(n_geo_pts, 1));
     L.gl = take(128 * sizeof(double));
+    L.geo_wave_end = take(sizeof(uint32_t) * std::max<uint64_t>(n_geo_items, 1));
     L.upload_end = o;
     L.counters = take(256);
     L.counts = take(256);
@@ -253,6 +254,8 @@ Layout make_layout(uint32_t n_req, uint64_t n_seg, uint64_t n_fin, uint64_t n_op
     L.ses = take(sizeof(double) * n_out);
     L.finals = take(sizeof(double) * std::max<uint64_t>(n_finals, 1));
     L.geo = take(sizeof(double) * std::max<uint64_t>(n_out, 1));
+    L.geo_panel = take(64 * sizeof(double) * std::max<uint64_t>(n_geo_waves, 1));
+    L.geo_pmax = take(sizeof(double) * std::max<uint64_t>(n_geo_waves, 1));
     L.total = o;
     return L;
 }
@@ -280,7 +283,7 @@ size_t request_device_bytes(const hexo_request* r) {
     if (!plan_request(r, p)) return 0;
     return sizeof(DevReq) + p.segs.size() * sizeof(DevSeg) + p.n_fins * sizeof(DevFin) + p.n_opts * sizeof(DevOpt) +
            32 + 16 + (size_t)16 * p.n_tiles * p.n_out + (size_t)16 * p.n_groups * p.n_out + 24 * p.n_out +
-           p.geo.items.size() * sizeof(DevGeo) + p.geo.pts.size() * sizeof(double);
+           p.geo.items.size() * (sizeof(DevGeo) + 4 + (kGeoMaxP0 + 1) * 520) + p.geo.pts.size() * sizeof(double);
 }
 
 size_t batch_device_bytes(const hexo_request* reqs, uint32_t n) {
@@ -314,7 +317,8 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         return HEXO_ERR_INVALID;
     }
     std::vector<ReqPlan> plans(n);
-    uint64_t n_seg = 0, n_fin = 0, n_opt = 0, n_partials = 0, n_groups = 0, n_out = 0, n_geo_items = 0, n_geo_pts = 0;
+    uint64_t n_seg = 0, n_fin = 0, n_opt = 0, n_partials = 0, n_groups = 0, n_out = 0;
+    uint64_t n_geo_items = 0, n_geo_pts = 0, n_geo_waves = 0;
     uint32_t n_cls[kPathClasses] = {};
     for (uint32_t i = 0; i < n; ++i) {
         std::string why = validate(&reqs[i]);
@@ -333,6 +337,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         }
         n_geo_items += p.geo.items.size();
         n_geo_pts += p.geo.pts.size() / 2;
+        for (const HostGeoItem& it : p.geo.items) n_geo_waves += it.p0 + 1;
         n_seg += p.segs.size();
         n_fin += p.n_fins;
         n_opt += p.n_opts;
@@ -350,7 +355,8 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     for (uint32_t cls = 1; cls < kPathClasses; ++cls) b->ppt[cls] = ctx->tuning.paths_per_thread;
     const uint64_t n_finals = finals_n ? *finals_n : 0;
     const Layout L =
-        make_layout(n, n_seg, n_fin, n_opt, n_cls, n_partials, n_groups, n_out, n_finals, n_geo_items, n_geo_pts);
+        make_layout(n, n_seg, n_fin, n_opt, n_cls, n_partials, n_groups, n_out, n_finals, n_geo_items, n_geo_pts,
+                    n_geo_waves);
 
     // host staging: the uploaded tables, then the result buffers
     b->upload_bytes = L.upload_end;
@@ -381,7 +387,8 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     DevGeo* hgi = reinterpret_cast<DevGeo*>(H + L.geo_items);
     double* hgp = reinterpret_cast<double*>(H + L.geo_pts);
     gauss_legendre64(reinterpret_cast<double*>(H + L.gl), reinterpret_cast<double*>(H + L.gl) + 64);
-    uint64_t geo_i = 0, geo_p = 0;
+    uint32_t* hgw = reinterpret_cast<uint32_t*>(H + L.geo_wave_end);
+    uint64_t geo_i = 0, geo_p = 0, geo_w = 0;
 
     const double L2E = 1.4426950408889634074;  // log2(e)
     uint64_t seg_i = 0, fin_i = 0, opt_i = 0, part_i = 0, grp_i = 0, out_i = 0, ge = 0;
@@ -492,6 +499,10 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
                 g.out = out_i + cvpay * p.n_opts + it.opt;
                 g.c = it.c;
                 g.h = it.h;
+                g.wave_begin = (uint32_t)geo_w;
+                g.p0 = it.p0;
+                geo_w += it.p0 + 1;
+                hgw[geo_i - 1] = (uint32_t)geo_w;
             }
             std::memcpy(hgp + 2 * geo_p, p.geo.pts.data(), sizeof(double) * p.geo.pts.size());
             geo_p += p.geo.pts.size() / 2;
@@ -583,6 +594,10 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     ga.gl = reinterpret_cast<const double*>(D + L.gl);
     ga.geo = reinterpret_cast<double*>(D + L.geo);
     ga.n_items = (uint32_t)geo_i;
+    ga.n_waves = (uint32_t)geo_w;
+    ga.wave_end = reinterpret_cast<const uint32_t*>(D + L.geo_wave_end);
+    ga.panel = reinterpret_cast<double*>(D + L.geo_panel);
+    ga.panel_max = reinterpret_cast<double*>(D + L.geo_pmax);
     ra.req_of = nullptr;
     ra.group_end = reinterpret_cast<const uint64_t*>(D + L.group_end);
     ra.out_end = reinterpret_cast<const uint64_t*>(D + L.out_end);

@@ -40,12 +40,14 @@ bool build_schedule(const double* T, uint32_t n_exp, const uint32_t* n_strikes, 
 // ---- geometric-average control variate (geo_host.cpp) -------------------------------------
 constexpr uint32_t kGeoMaxPanels = 4096;   // 64-node panels per exact price before it reads NaN
 constexpr uint32_t kGeoMaxPoints = 65536;  // grid steps of one control-variate expiry
+constexpr uint32_t kGeoMaxP0 = 128;        // panels per item evaluated in parallel
 struct HostGeoItem {
     uint32_t pts_begin = 0;  // request-relative (t_j, v_j) pair index
     uint32_t n_pts = 0;
     uint32_t n_strikes = 0;
     uint32_t opt = 0;  // request-relative first option of the block
     uint32_t max_panels = 0;
+    uint32_t p0 = 0;  // panels evaluated in parallel
     double c = 0.0, h = 0.0;
 };
 struct GeoPlan {

@@ -7,16 +7,19 @@
 // by a backward recursion over the grid points: at t_j the log-price coefficient a grows by s v_j;
 // over (t_{j-1}, t_j] the Riccati equations of exp(A + a y + B V),
 //   B' = sigma^2 B^2 / 2 + (rho sigma a - kappa) B + (a^2 - a) / 2,   A' = kappa theta B + r a,
-// are solved in closed form from the running B (no cancellation as sigma -> 0, see geo_step).
+// are solved in closed form from the running B (no cancellation as sigma -> 0, see geo_cf).
 // The call price is Lewis' inversion
 //   E[(c e^L - K)^+] = c (phi(1) - e^{k/2}/pi int_0^inf Re(e^{iuk} phi(1/2 - iu)) / (u^2 + 1/4) du),
 // k = ln(K / c), by 64-node Gauss-Legendre panels of width h (host-chosen from the scale of L and
 // the strikes) until the characteristic function has decayed below 1e-15 on a whole panel.
 //
-// One wavefront per (request, expiry, block of <= 64 strikes): lane q evaluates node q of the
-// current panel (a recursion over the expiry's grid, a few hundred to a few thousand steps); the
-// panel's values go through LDS and lane i accumulates strike i's integral over the 64 nodes in
-// node order, so every price is a fixed-order sum. Runs once per batch, before the path kernels.
+// Work items are (request, expiry, block of <= 64 strikes). Pass 1 evaluates the first p0 panels
+// of every item at once, one wavefront per panel (lane q: node q, a recursion over the expiry's
+// grid, a few hundred to a few thousand steps; lane i: strike i's panel sum over the 64 nodes in
+// node order), plus E[e^L]; p0 is the host's estimate of where the characteristic function has
+// decayed. Pass 2 gives each item one wavefront: lane i adds strike i's panel sums in panel order
+// and evaluates further panels itself if the first p0 did not reach the decay; every price is
+// therefore the same fixed-order sum whatever p0 is. Runs once per batch, before the path kernels.
 #include <hip/hip_runtime.h>
 
 #include "cplx_device.hpp"
@@ -65,53 +68,117 @@ __device__ Cplx geo_cf(const DevReq& R, const double* pts, uint32_t n, Cplx s) {
     return cexp(A + R.v0 * B);
 }
 
-__global__ __launch_bounds__(256) void k_geo_exact(GeoArgs a) {
-    __shared__ double sh[kGeoWaves][3][64];  // per wave: Re phi, Im phi (times the node weight), u
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) x = fmax(x, __shfl_xor(x, m));
+    return x;  // every lane holds the wave's maximum
+}
+
+__device__ __forceinline__ uint32_t find_item(const uint32_t* end, uint32_t n, uint32_t t) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (end[mid] > t)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// Panel p of an item, by one wavefront: lane q evaluates node q, u_q = (p + (x_q + 1)/2) h, and
+// phi(1/2 - i u_q) weighted by w_q (h/2) / (u_q^2 + 1/4); the 64 values go through LDS (sh: this
+// wave's 3 x 64 doubles) and lane i returns strike i's panel sum, node by node, of
+// Re(e^{i u_q k_i} weighted phi). *mag is the wave's max |phi| over the panel.
+__device__ double panel_sum(const GeoArgs& a, const DevGeo& G, const DevReq& R, const double* pts, uint32_t p, int lane,
+                            double k, bool has_k, double (*sh)[64], double* mag) {
+    const double u = ((double)p + 0.5 * (a.gl[lane] + 1.0)) * G.h;
+    const Cplx ph = geo_cf(R, pts, G.n_pts, cx(0.5, -u));
+    const double f = 0.5 * G.h * a.gl[64 + lane] / (u * u + 0.25);
+    sh[0][lane] = f * ph.re;
+    sh[1][lane] = f * ph.im;
+    sh[2][lane] = u;
+    *mag = wave_max(cabs_(ph));
+    wave_sync_lds();
+    double sum = 0.0;
+    if (has_k) {
+        for (int q = 0; q < 64; ++q) {
+            double sn, cs;
+            sincos(sh[2][q] * k, &sn, &cs);
+            sum += cs * sh[0][q] - sn * sh[1][q];
+        }
+    }
+    wave_sync_lds();  // sh is rewritten by the next panel
+    return sum;
+}
+
+__device__ __forceinline__ void strike_of(const GeoArgs& a, const DevGeo& G, int lane, double& K, double& k,
+                                          bool& has_k) {
+    const bool in = (uint32_t)lane < G.n_strikes;
+    K = in ? a.opts[G.opt + (uint32_t)lane].k_abs : 0.0;
+    has_k = K > 0.0;  // K <= 0: max(G - K, 0) = G - K needs no integral
+    k = has_k ? log(K / G.c) : 0.0;
+}
+
+// Pass 1: every wavefront evaluates one panel of one item (all panels of all items at once) and
+// stores each strike's panel sum; the item's last wavefront evaluates E[e^L] instead.
+__global__ __launch_bounds__(256) void k_geo_panels(GeoArgs a) {
+    __shared__ double sh[kGeoWaves][3][64];
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
+    const uint32_t wv = blockIdx.x * kGeoWaves + (uint32_t)w;
+    if (wv >= a.n_waves) return;  // whole wavefronts only
+    const uint32_t item = __builtin_amdgcn_readfirstlane(find_item(a.wave_end, a.n_items, wv));
+    const DevGeo& G = a.items[item];
+    const DevReq& R = a.reqs[G.req];
+    const double* pts = a.pts + 2 * (size_t)G.pts_begin;
+    const uint32_t p = wv - G.wave_begin;
+    if (p == G.p0) {
+        const Cplx e1 = geo_cf(R, pts, G.n_pts, cx(1.0));
+        if (lane == 0) a.panel_max[wv] = e1.re;
+        return;
+    }
+    double K, k, mag;
+    bool has_k;
+    strike_of(a, G, lane, K, k, has_k);
+    a.panel[(size_t)wv * 64 + (uint32_t)lane] = panel_sum(a, G, R, pts, p, lane, k, has_k, sh[w], &mag);
+    if (lane == 0) a.panel_max[wv] = mag;
+}
+
+// Pass 2: one wavefront per item; lane i adds strike i's panel sums in panel order up to the first
+// panel (after the first) whose |phi| stays below 1e-15, evaluating further panels itself if the
+// parallel pass did not reach it; the price is the same sum whatever p0 is.
+__global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a) {
+    __shared__ double sh[kGeoWaves][3][64];
     const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
     const uint32_t item = blockIdx.x * kGeoWaves + (uint32_t)w;
     if (item >= a.n_items) return;  // whole wavefronts only
     const DevGeo& G = a.items[item];
     const DevReq& R = a.reqs[G.req];
     const double* pts = a.pts + 2 * (size_t)G.pts_begin;
-    const bool has_k = (uint32_t)lane < G.n_strikes;
-    const double K = has_k ? a.opts[G.opt + (uint32_t)lane].k_abs : 0.0;
-    const double k = K > 0.0 ? log(K / G.c) : 0.0;
-    const Cplx e1 = geo_cf(R, pts, G.n_pts, cx(1.0));  // E[e^L]
-    const double xq = a.gl[lane], wq = a.gl[64 + lane];
+    double K, k;
+    bool has_k;
+    strike_of(a, G, lane, K, k, has_k);
+    const double e1 = a.panel_max[G.wave_begin + G.p0];  // E[e^L]
     double integ = 0.0;
     bool converged = false;
-    for (uint32_t p = 0; p < G.max_panels; ++p) {
-        const double u = ((double)p + 0.5 * (xq + 1.0)) * G.h;
-        const Cplx ph = geo_cf(R, pts, G.n_pts, cx(0.5, -u));
-        const double f = 0.5 * G.h * wq / (u * u + 0.25);
-        sh[w][0][lane] = f * ph.re;
-        sh[w][1][lane] = f * ph.im;
-        sh[w][2][lane] = u;
-        double amax = cabs_(ph);
-#pragma unroll
-        for (int m = 1; m < 64; m <<= 1) amax = fmax(amax, __shfl_xor(amax, m));
-        wave_sync_lds();
-        if (K > 0.0) {
-            for (int q = 0; q < 64; ++q) {
-                double sn, cs;
-                sincos(sh[w][2][q] * k, &sn, &cs);
-                integ += cs * sh[w][0][q] - sn * sh[w][1][q];
-            }
-        }
-        wave_sync_lds();
-        if (p > 0 && amax < 1e-15) {  // wave-uniform: every lane holds the wave's maximum
-            converged = true;
-            break;
-        }
+    uint32_t p = 0;
+    for (; p < G.p0 && !converged; ++p) {
+        integ += a.panel[(size_t)(G.wave_begin + p) * 64 + (uint32_t)lane];
+        converged = p > 0 && a.panel_max[G.wave_begin + p] < 1e-15;
     }
-    if (!has_k) return;
+    for (; p < G.max_panels && !converged; ++p) {
+        double mag;
+        integ += panel_sum(a, G, R, pts, p, lane, k, has_k, sh[w], &mag);
+        converged = p > 0 && mag < 1e-15;  // wave-uniform
+    }
+    if ((uint32_t)lane >= G.n_strikes) return;
     double price;
     if (!converged)
         price = __builtin_nan("");
-    else if (K > 0.0)
-        price = G.c * (e1.re - exp(0.5 * k) / kPi * integ);
+    else if (has_k)
+        price = G.c * (e1 - exp(0.5 * k) / kPi * integ);
     else
-        price = G.c * e1.re - K;  // max(G - K, 0) = G - K
+        price = G.c * e1 - K;  // max(G - K, 0) = G - K
     a.geo[G.out + (uint64_t)lane] = price;
 }
 
@@ -119,7 +186,10 @@ __global__ __launch_bounds__(256) void k_geo_exact(GeoArgs a) {
 
 hipError_t launch_geo(const GeoArgs& a, hipStream_t st) {
     if (a.n_items == 0) return hipSuccess;
-    k_geo_exact<<<(a.n_items + kGeoWaves - 1) / kGeoWaves, 64 * kGeoWaves, 0, st>>>(a);
+    k_geo_panels<<<(a.n_waves + kGeoWaves - 1) / kGeoWaves, 64 * kGeoWaves, 0, st>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_geo_sum<<<(a.n_items + kGeoWaves - 1) / kGeoWaves, 64 * kGeoWaves, 0, st>>>(a);
     return hipGetLastError();
 }
 

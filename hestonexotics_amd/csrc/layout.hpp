@@ -106,6 +106,8 @@ struct DevGeo {
     uint32_t n_strikes;   // strikes of this block (<= 64)
     uint32_t opt;         // first option of the block (batch-global DevOpt index)
     uint32_t max_panels;  // bound on 64-node panels (the price is NaN if not converged by then)
+    uint32_t wave_begin;  // first wavefront of the item in the parallel panel pass
+    uint32_t p0;          // panels evaluated in parallel (that pass has p0 + 1 waves: the last one E[e^L])
     uint64_t out;         // geo[] index of the block's first control-variate output
     double c;             // W S0
     double h;             // width in u of one panel
@@ -119,6 +121,10 @@ struct GeoArgs {
     const double* gl;   // 64 Gauss-Legendre nodes on [-1, 1], then their 64 weights
     double* geo;        // exact geometric prices (undiscounted, absolute units), per output
     uint32_t n_items;
+    uint32_t n_waves;       // waves of the parallel panel pass (sum over items of p0 + 1)
+    const uint32_t* wave_end;  // inclusive prefix over items of p0 + 1
+    double* panel;          // per panel wave: each strike's panel sum (64 doubles)
+    double* panel_max;      // per panel wave: max |phi| over its nodes; E[e^L] in the item's last wave
 };
 
 struct ReduceArgs {
