@@ -332,9 +332,10 @@ def main():
                     help="profiling driver: --profile-passes eager FP32 passes of the workload, then a plain copy of "
                          "--copy-mib MiB (the known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--profile-passes", type=int, default=20)
-    ap.add_argument("--profile-workload", choices=("c2", "c3", "c5", "c2cv", "c2mart"), default="c2",
+    ap.add_argument("--profile-workload", choices=("c2", "c3", "c5", "c2cv", "c3cv", "c2mart"), default="c2",
                     help="workload of --profile: the grid (c2), the 512 uneven requests (c3), the stiff request (c5), "
-                         "the grid with the control-variate Asian (c2cv) or with the martingale correction (c2mart)")
+                         "the grid or the 512 requests with the control-variate Asian (c2cv, c3cv), the grid with the "
+                         "martingale correction (c2mart)")
     ap.add_argument("--profile-swift", action="store_true",
                     help="profiling driver for the SWIFT pricer: --profile-passes calls on the calibration batch")
     ap.add_argument("--copy-mib", type=int, default=512)
@@ -363,6 +364,7 @@ def main():
                      "c3": lambda: W.c3(precision=H.FP32),
                      "c5": lambda: [W.c5(precision=H.FP32)],
                      "c2cv": lambda: with_(W.c2(n_paths=args.paths, precision=H.FP32), payoffs=H.ASIAN_GEO_CV),
+                     "c3cv": lambda: with_(W.c3(precision=H.FP32, payoffs=H.ASIAN), payoffs=H.ASIAN_GEO_CV),
                      "c2mart": lambda: with_(W.c2(n_paths=args.paths, precision=H.FP32), flags=H.MARTINGALE),
                      }[args.profile_workload]()
         bp = ctx.batch(prof_reqs)

@@ -143,10 +143,10 @@ bool plan_geo(const hexo_request* r, const std::vector<HostSeg>& segs, GeoPlan& 
                 it.h = std::min(1.0 / sd, 8.0 * M_PI / (kmax + 4.0 * sd + 1e-3));
                 // bounded work per wavefront: panels x grid steps <= 2^26 recursion steps
                 it.max_panels = (uint32_t)std::min<uint64_t>(kGeoMaxPanels, (1ull << 26) / (uint64_t)n);
-                // panels evaluated in parallel: up to four times the u where a Gaussian L's
+                // panels evaluated in parallel: up to ten times the u where a Gaussian L's
                 // characteristic function falls below 1e-16 (Heston's decays slower; the sum pass
-                // continues by itself where needed)
-                const double u_max = 4.0 * std::sqrt(2.0 * 36.9 / std::max(var, 1e-12));
+                // continues by itself where needed, and stops at the first decayed panel)
+                const double u_max = 10.0 * std::sqrt(2.0 * 36.9 / std::max(var, 1e-12));
                 it.p0 = (uint32_t)std::min<double>(std::min<uint32_t>(kGeoMaxP0, it.max_panels),
                                                    std::max(2.0, std::ceil(u_max / it.h)));
                 g.items.push_back(it);
