@@ -124,6 +124,22 @@ def test_fp32_control_variate_vs_fp64_and_variance_reduction(ctx):
     assert max(ratios) < 0.1
 
 
+@pytest.mark.parametrize("name", sorted(geo_requests()))
+def test_fp32_control_variate_close_to_fp64(ctx, name):
+    """FP32 kernels (plain and compensated reduction, with and without the martingale
+    correction) against FP64 on identical draws, every request shape: multi-expiry runs exercise
+    the log-price accumulation across step-width changes."""
+    say("gpu geo cv fp32 vs fp64", name)
+    for flags in (0, H.MARTINGALE):
+        p64, _ = ctx.price(cv(geo_requests()[name], H.FP64, H.ASIAN | H.ASIAN_GEO_CV, flags))
+        for prec in (H.FP32, H.FP32_COMPENSATED):
+            p32, _ = ctx.price(cv(geo_requests()[name], prec, H.ASIAN | H.ASIAN_GEO_CV, flags))
+            big = np.abs(p64) > 1e-2
+            rel = np.abs(p32[big] - p64[big]) / np.abs(p64[big])
+            say(f"  flags {flags} precision {prec}: max rel {rel.max():.2e}")
+            assert rel.max() < 1e-4
+
+
 def test_mixed_batch_equals_single(ctx):
     say("gpu geo cv mixed batch == single")
     base = geo_requests()["c1_like"]
