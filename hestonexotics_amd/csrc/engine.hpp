@@ -40,6 +40,7 @@ bool build_schedule(const double* T, uint32_t n_exp, const uint32_t* n_strikes, 
 // ---- geometric-average control variate (geo_host.cpp) -------------------------------------
 constexpr uint32_t kGeoMaxPanels = 4096;   // 64-node panels per exact price before it reads NaN
 constexpr uint32_t kGeoMaxPoints = 65536;  // grid steps of one control-variate expiry
+constexpr uint64_t kGeoMaxRequestPoints = 1u << 22;  // grid points over a request's control-variate expiries
 constexpr uint32_t kGeoMaxP0 = 128;        // panels per item evaluated in parallel
 struct HostGeoItem {
     uint32_t pts_begin = 0;  // request-relative (t_j, v_j) pair index

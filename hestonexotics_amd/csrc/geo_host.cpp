@@ -46,10 +46,12 @@ void gauss_legendre64(double* x, double* w) {
     }
 }
 
-int64_t functional_weights(const std::vector<HostSeg>& segs, uint32_t flags, uint32_t chain, std::vector<double>& t,
-                           std::vector<double>& w) {
-    std::vector<double> acc{0.0};
-    std::vector<double> tt{0.0};
+// One pass over the run schedule: the accumulation weights and grid times as the path kernels
+// build them, and at every finalisation fn(F, p, t, w) with the finalised functional's weights
+// w[0..p] (divided by T).
+template <typename Fn>
+static void walk_functionals(const std::vector<HostSeg>& segs, uint32_t flags, Fn fn) {
+    std::vector<double> acc{0.0}, tt{0.0}, w;
     uint64_t p = 0;
     for (const HostSeg& S : segs) {
         for (uint32_t i = 1; i <= S.n_steps; ++i) {
@@ -62,7 +64,6 @@ int64_t functional_weights(const std::vector<HostSeg>& segs, uint32_t flags, uin
             }
         }
         for (const HostFin& F : S.fins) {
-            if (F.chain != chain) continue;
             w = acc;
             if (flags & HEXO_TRAPEZOID) {
                 const double h = 0.5 * (F.T - F.tp);
@@ -73,86 +74,104 @@ int64_t functional_weights(const std::vector<HostSeg>& segs, uint32_t flags, uin
                 w[p - 1] -= F.w;
             }
             for (double& x : w) x /= F.T;
-            t = tt;
-            return (int64_t)p;
+            if (!fn(F, p, tt, w)) return;
         }
         // the accumulation after the run's last step uses the step width that follows it
         acc[p - 1] += 0.5 * S.dnext;
         acc[p] += 0.5 * S.dnext;
     }
-    return -1;
+}
+
+int64_t functional_weights(const std::vector<HostSeg>& segs, uint32_t flags, uint32_t chain, std::vector<double>& t,
+                           std::vector<double>& w) {
+    int64_t n = -1;
+    walk_functionals(segs, flags, [&](const HostFin& F, uint64_t p, const std::vector<double>& tt,
+                                      const std::vector<double>& ww) {
+        if (F.chain != chain) return true;
+        t = tt;
+        w = ww;
+        n = (int64_t)p;
+        return false;
+    });
+    return n;
 }
 
 bool plan_geo(const hexo_request* r, const std::vector<HostSeg>& segs, GeoPlan& g, std::string& why) {
     g = GeoPlan();
     const bool cv = (r->payoffs & HEXO_ASIAN_GEO_CV) != 0;
+    if (!cv) {
+        for (const HostSeg& S : segs) g.fin_W.insert(g.fin_W.end(), S.fins.size(), 1.0);
+        return true;
+    }
     std::vector<uint32_t> opt_begin(r->n_expiries + 1, 0);
     for (uint32_t c = 0; c < r->n_expiries; ++c) opt_begin[c + 1] = opt_begin[c] + r->n_strikes[c];
-    for (const HostSeg& S : segs)
-        for (const HostFin& F : S.fins) {
-            if (!cv || r->n_strikes[F.chain] == 0) {
-                g.fin_W.push_back(1.0);
-                continue;
-            }
-            std::vector<double> t, w;
-            const int64_t n = functional_weights(segs, r->flags, F.chain, t, w);
-            double W = 0.0;
-            for (double x : w) W += x;
-            if (n < 1 || !(W >= 0.25)) {
-                why = "geometric control variate: the Asian average of expiry " + std::to_string(F.chain) +
-                      " has weights summing to " + std::to_string(W) + " (< 0.25; use at least 2 steps)";
-                return false;
-            }
-            if (n > (int64_t)kGeoMaxPoints) {
-                why = "geometric control variate: expiry " + std::to_string(F.chain) + " has " + std::to_string(n) +
-                      " grid steps (at most " + std::to_string(kGeoMaxPoints) + ")";
-                return false;
-            }
-            g.fin_W.push_back(W);
-            const uint32_t pts_begin = (uint32_t)(g.pts.size() / 2);
-            // scale of L = sum_j v_j ln(X_{t_j}/S0): mean and variance along E[V_t] (panel width only)
-            double mu = 0.0, var = 0.0, suffix = 0.0;
-            for (int64_t j = n; j >= 1; --j) {
-                suffix += w[j] / W;
-                const double tm = 0.5 * (t[j] + t[j - 1]);
-                const double ev = r->theta + (r->v0 - r->theta) * std::exp(-r->kappa * tm);
-                const double rd = (r->flags & HEXO_DRIFT) ? r->r : 0.0;
-                mu += (t[j] - t[j - 1]) * (rd - 0.5 * ev) * suffix;
-                var += (t[j] - t[j - 1]) * ev * suffix * suffix;
-            }
-            for (int64_t j = 0; j <= n; ++j) {
-                g.pts.push_back(t[j]);
-                g.pts.push_back(w[j] / W);
-            }
-            const double sd = std::sqrt(std::max(var, 1e-12));
-            const double c = W * r->S0;
-            const uint32_t ns = r->n_strikes[F.chain];
-            for (uint32_t kb = 0; kb < ns; kb += 64) {
-                HostGeoItem it;
-                it.pts_begin = pts_begin;
-                it.n_pts = (uint32_t)n;
-                it.n_strikes = std::min<uint32_t>(64, ns - kb);
-                it.opt = opt_begin[F.chain] + kb;
-                it.c = c;
-                double kmax = 0.0;
-                for (uint32_t i = 0; i < it.n_strikes; ++i) {
-                    const double K = r->strikes[it.opt + i];
-                    if (K > 0.0) kmax = std::max(kmax, std::fabs(std::log(K / c) - mu));
-                }
-                // a 64-node panel spans at most 1/sd in u and four periods of e^{iu(k - mu)}
-                it.h = std::min(1.0 / sd, 8.0 * M_PI / (kmax + 4.0 * sd + 1e-3));
-                // bounded work per wavefront: panels x grid steps <= 2^26 recursion steps
-                it.max_panels = (uint32_t)std::min<uint64_t>(kGeoMaxPanels, (1ull << 26) / (uint64_t)n);
-                // panels evaluated in parallel: up to ten times the u where a Gaussian L's
-                // characteristic function falls below 1e-16 (Heston's decays slower; the sum pass
-                // continues by itself where needed, and stops at the first decayed panel)
-                const double u_max = 10.0 * std::sqrt(2.0 * 36.9 / std::max(var, 1e-12));
-                it.p0 = (uint32_t)std::min<double>(std::min<uint32_t>(kGeoMaxP0, it.max_panels),
-                                                   std::max(2.0, std::ceil(u_max / it.h)));
-                g.items.push_back(it);
-            }
+    const double rd = (r->flags & HEXO_DRIFT) ? r->r : 0.0;
+    bool ok = true;
+    walk_functionals(segs, r->flags, [&](const HostFin& F, uint64_t p, const std::vector<double>& t,
+                                        const std::vector<double>& w) {
+        if (r->n_strikes[F.chain] == 0) {
+            g.fin_W.push_back(1.0);
+            return true;
         }
-    return true;
+        const int64_t n = (int64_t)p;
+        double W = 0.0;
+        for (int64_t j = 0; j <= n; ++j) W += w[j];
+        if (n < 1 || !(W >= 0.25)) {
+            why = "geometric control variate: the Asian average of expiry " + std::to_string(F.chain) +
+                  " has weights summing to " + std::to_string(W) + " (< 0.25; use at least 2 steps)";
+            return ok = false;
+        }
+        if (n > (int64_t)kGeoMaxPoints || g.pts.size() / 2 + (size_t)n + 1 > kGeoMaxRequestPoints) {
+            why = "geometric control variate: expiry " + std::to_string(F.chain) + " has " + std::to_string(n) +
+                  " grid steps (at most " + std::to_string(kGeoMaxPoints) + " per expiry and " +
+                  std::to_string(kGeoMaxRequestPoints) + " over a request's expiries)";
+            return ok = false;
+        }
+        g.fin_W.push_back(W);
+        const uint32_t pts_begin = (uint32_t)(g.pts.size() / 2);
+        // scale of L = sum_j v_j ln(X_{t_j}/S0): mean and variance along E[V_t] (panel width only)
+        double mu = 0.0, var = 0.0, suffix = 0.0;
+        for (int64_t j = n; j >= 1; --j) {
+            suffix += w[j] / W;
+            const double tm = 0.5 * (t[j] + t[j - 1]);
+            const double ev = r->theta + (r->v0 - r->theta) * std::exp(-r->kappa * tm);
+            mu += (t[j] - t[j - 1]) * (rd - 0.5 * ev) * suffix;
+            var += (t[j] - t[j - 1]) * ev * suffix * suffix;
+        }
+        for (int64_t j = 0; j <= n; ++j) {
+            g.pts.push_back(t[j]);
+            g.pts.push_back(w[j] / W);
+        }
+        const double sd = std::sqrt(std::max(var, 1e-12));
+        const double c = W * r->S0;
+        const uint32_t ns = r->n_strikes[F.chain];
+        for (uint32_t kb = 0; kb < ns; kb += 64) {
+            HostGeoItem it;
+            it.pts_begin = pts_begin;
+            it.n_pts = (uint32_t)n;
+            it.n_strikes = std::min<uint32_t>(64, ns - kb);
+            it.opt = opt_begin[F.chain] + kb;
+            it.c = c;
+            double kmax = 0.0;
+            for (uint32_t i = 0; i < it.n_strikes; ++i) {
+                const double K = r->strikes[it.opt + i];
+                if (K > 0.0) kmax = std::max(kmax, std::fabs(std::log(K / c) - mu));
+            }
+            // a 64-node panel spans at most 1/sd in u and four periods of e^{iu(k - mu)}
+            it.h = std::min(1.0 / sd, 8.0 * M_PI / (kmax + 4.0 * sd + 1e-3));
+            // bounded work per wavefront: panels x grid steps <= 2^26 recursion steps
+            it.max_panels = (uint32_t)std::min<uint64_t>(kGeoMaxPanels, (1ull << 26) / (uint64_t)n);
+            // panels evaluated in parallel: up to ten times the u where a Gaussian L's
+            // characteristic function falls below 1e-16 (Heston's decays slower; the sum pass
+            // continues by itself where needed, and stops at the first decayed panel)
+            const double u_max = 10.0 * std::sqrt(2.0 * 36.9 / std::max(var, 1e-12));
+            it.p0 = (uint32_t)std::min<double>(std::min<uint32_t>(kGeoMaxP0, it.max_panels),
+                                               std::max(2.0, std::ceil(u_max / it.h)));
+            g.items.push_back(it);
+        }
+        return true;
+    });
+    return ok;
 }
 
 }  // namespace hexo
