@@ -21,8 +21,10 @@ namespace {
 const Json kNull{};
 
 struct Parser {
+    static constexpr int kMaxDepth = 256;  // nesting bound (the API's responses nest 3 deep)
     const std::string& s;
     size_t i = 0;
+    int depth = 0;
     explicit Parser(const std::string& t) : s(t) {}
     [[noreturn]] void fail(const char* what) const {
         throw std::runtime_error(std::string("JSON: ") + what + " at offset " + std::to_string(i));
@@ -53,7 +55,12 @@ struct Parser {
                     case 'f': c = '\f'; break;
                     case 'u': {
                         if (i + 4 > s.size()) fail("bad unicode escape");
-                        const unsigned cp = (unsigned)std::stoul(s.substr(i, 4), nullptr, 16);
+                        unsigned cp = 0;
+                        for (int d = 0; d < 4; ++d) {
+                            const char h = s[i + d];
+                            if (!std::isxdigit((unsigned char)h)) fail("bad unicode escape");
+                            cp = cp * 16 + (unsigned)(h <= '9' ? h - '0' : (h | 0x20) - 'a' + 10);
+                        }
                         i += 4;
                         c = cp < 0x80 ? (char)cp : '?';  // field values here are ASCII
                         break;
@@ -70,6 +77,12 @@ struct Parser {
     Json value() {
         ws();
         if (i >= s.size()) fail("unexpected end");
+        if (++depth > kMaxDepth) fail("nesting too deep");
+        Json v = value_body();
+        --depth;
+        return v;
+    }
+    Json value_body() {
         Json v;
         const char c = s[i];
         if (c == '{') {

@@ -53,3 +53,17 @@ def test_malformed_market_data(tmp_path):
     assert out.returncode == 2 and "JSON" in out.stderr
     out = run("-d", "MISSING", data=str(tmp_path))
     assert out.returncode == 2
+
+
+@pytest.mark.parametrize("text", [
+    "[" * 100_000,                                            # nesting far beyond any API response
+    '{"quotes": {"quote": {"bid": "\\uZZZZ"}}}',              # bad unicode escape
+    '{"quotes": {"quote": {"bid": 10.0}}} trailing',         # accepted prefix, then junk is ignored
+    '{"quotes": {"quote": {"bid": 10.0',                      # truncated
+])
+def test_hostile_json_is_an_error_not_a_crash(tmp_path, text):
+    (tmp_path / "EVIL").mkdir()
+    (tmp_path / "EVIL" / "quote.json").write_text(text)
+    (tmp_path / "EVIL" / "expirations.json").write_text('{"expirations": {"date": []}}')
+    out = run("-d", "EVIL", data=str(tmp_path))
+    assert out.returncode in (0, 2), (out.returncode, out.stderr[-300:])
