@@ -253,7 +253,7 @@ def control_variate_leg(ctx, args, plain_value: float, plain_kernel_ms: float):
 
     reqs = W.c2(n_paths=args.paths, precision=H.FP32)
     for q in reqs:
-        q.payoffs = H.ASIAN | H.EUROPEAN | H.ASIAN_GEO_CV
+        q.payoffs = H.ASIAN | H.EUROPEAN | H.ASIAN_GEO_CV | H.GEO_ASIAN
     nominal = W.path_steps(reqs)
     steps = max(2, args.steps // 3)
     b = ctx.batch(reqs)
@@ -261,26 +261,31 @@ def control_variate_leg(ctx, args, plain_value: float, plain_kernel_ms: float):
     pall, sall = b.results()
     res = list(zip(b.per_request(pall), b.per_request(sall)))
     b.close()
-    factors, atm = [], []
-    for q, (p, s) in zip(reqs, res):
+    factors, atm, opt_factors = [], [], []
+    t0 = time.perf_counter()
+    exact = [ctx.geo_asian_price(q) for q in reqs]  # exact geometric prices (each its own batch)
+    geo_ms = 1e3 * (time.perf_counter() - t0) / len(reqs)
+    for q, (p, s), e in zip(reqs, res, exact):
         d = q.split(s)
         pr = q.split(p)
         ok = (pr["asian"] > 1e-2) & (d["asian_geo_cv"] > 0)
         factors.extend(list((d["asian"][ok] / d["asian_geo_cv"][ok]) ** 2))
         k = int(np.argmin(np.abs(q.flat_strikes() - q.S0)))
         atm.append(float((d["asian"][k] / d["asian_geo_cv"][k]) ** 2))
-    # the exact geometric prices: one wavefront per expiry, once per batch
-    t0 = time.perf_counter()
-    for q in reqs:
-        ctx.geo_asian_price(q)
-    geo_ms = 1e3 * (time.perf_counter() - t0) / len(reqs)
+        # the regression-coefficient estimator from the geometric block
+        _, se_opt, _ = H.cv_optimal((pr["asian"], d["asian"]), (pr["geo_asian"], d["geo_asian"]),
+                                    (pr["asian_geo_cv"], d["asian_geo_cv"]), e)
+        okk = ok & (se_opt > 0)
+        opt_factors.extend(list((d["asian"][okk] / se_opt[okk]) ** 2))
     value = nominal * steps / wall
-    return {"workload": "C2 grid, Asian + European + control-variate Asian from the same FP32 paths",
+    return {"workload": "C2 grid, Asian + European + control-variate Asian + geometric from the same FP32 paths",
             "value": value, "kernel_ms": kms, "over_plain": value / plain_value,
             "kernel_ms_plain": plain_kernel_ms,
             "variance_reduction_median": float(np.median(factors)),
             "variance_reduction_min": float(np.min(factors)),
             "variance_reduction_atm_by_maturity": atm,
+            "variance_reduction_regression_coefficient": {"median": float(np.median(opt_factors)),
+                                                           "min": float(np.min(opt_factors))},
             "exact_price_ms_per_request": geo_ms,
             "effective_path_steps_per_s_median": value * float(np.median(factors))}
 

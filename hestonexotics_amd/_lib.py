@@ -16,7 +16,7 @@ ERRORS = {
     -5: "HEXO_ERR_ADMISSION", -6: "HEXO_ERR_NOT_FOUND", -7: "HEXO_ERR_CANCELLED", -8: "HEXO_ERR_TIMEOUT",
     -9: "HEXO_ERR_STATE", -10: "HEXO_ERR_INJECTED",
 }
-ASIAN, EUROPEAN, ASIAN_GEO_CV = 1, 2, 4
+ASIAN, EUROPEAN, ASIAN_GEO_CV, GEO_ASIAN = 1, 2, 4, 8
 DRIFT, DISCOUNT, TRAPEZOID, MARTINGALE = 1, 2, 4, 8
 FP64, FP32, FP32_COMPENSATED = 0, 1, 2
 

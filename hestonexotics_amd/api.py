@@ -17,13 +17,13 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ASIAN, ASIAN_GEO_CV, CALIB_INITIAL_GIVEN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED,
-                   FP64, MARTINGALE, SWIFT_FIXED_GRID, TRAPEZOID, HexoError, hexo_batch_stats, hexo_calib_chain,
+                   FP64, GEO_ASIAN, MARTINGALE, SWIFT_FIXED_GRID, TRAPEZOID, HexoError, hexo_batch_stats, hexo_calib_chain,
                    hexo_calib_config, hexo_calib_result, hexo_request, hexo_server_config, hexo_server_stats,
                    hexo_swift_grid, hexo_swift_request, hexo_ticket_info, hexo_tuning)
 
-__all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "ASIAN_GEO_CV", "DRIFT", "DISCOUNT", "TRAPEZOID",
-           "MARTINGALE", "FP64", "FP32", "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "asian_weights",
-           "SwiftChain", "SwiftCall", "swift_grid"]
+__all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "ASIAN_GEO_CV", "GEO_ASIAN", "DRIFT", "DISCOUNT",
+           "TRAPEZOID", "MARTINGALE", "FP64", "FP32", "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "asian_weights",
+           "SwiftChain", "SwiftCall", "swift_grid", "cv_optimal"]
 
 GRID_FIELDS = ("m", "exp2_m", "sqrt_exp2_m", "lower", "upper", "k1", "k2", "J")
 
@@ -103,7 +103,7 @@ class Request:
 
     @property
     def n_payoffs(self) -> int:
-        return bin(self.payoffs & (ASIAN | EUROPEAN | ASIAN_GEO_CV)).count("1")
+        return bin(self.payoffs & (ASIAN | EUROPEAN | ASIAN_GEO_CV | GEO_ASIAN)).count("1")
 
     @property
     def n_outputs(self) -> int:
@@ -135,11 +135,12 @@ class Request:
         return q
 
     def split(self, prices: np.ndarray) -> dict:
-        """Outputs are payoff-major: the Asian, European and control-variate Asian blocks, those
-        priced, in that order."""
+        """Outputs are payoff-major: the Asian, European, control-variate Asian and geometric
+        blocks, those priced, in that order."""
         out, o = {}, 0
         n = self.n_opts
-        for bit, name in ((ASIAN, "asian"), (EUROPEAN, "european"), (ASIAN_GEO_CV, "asian_geo_cv")):
+        for bit, name in ((ASIAN, "asian"), (EUROPEAN, "european"), (ASIAN_GEO_CV, "asian_geo_cv"),
+                          (GEO_ASIAN, "geo_asian")):
             if self.payoffs & bit:
                 out[name] = prices[o:o + n]
                 o += n
@@ -160,6 +161,23 @@ def _dp(a: np.ndarray):
 def validate(req: Request) -> int:
     lib = _lib.load()
     return lib.hexo_validate_request(C.byref(req.to_c()))
+
+
+def cv_optimal(asian, geo, cv, geo_exact):
+    """The control variate with the regression coefficient beta = Cov(A, G) / Var(G) instead of 1.
+
+    asian, geo, cv: (prices, std_errs) of the ASIAN, GEO_ASIAN and ASIAN_GEO_CV blocks of one
+    request (same paths, same discounting); geo_exact: its exact geometric prices
+    (Context.geo_asian_price). Var(A - G) = Var A + Var G - 2 Cov(A, G) gives the covariance; the
+    estimator A - beta (G - E[G]) has variance Var A (1 - corr^2), never above the plain one.
+    Returns (prices, std_errs, beta)."""
+    (pa, sa), (pg, sg), (_, sd) = [(np.asarray(p, np.float64), np.asarray(s, np.float64))
+                                   for p, s in (asian, geo, cv)]
+    cov = 0.5 * (sa * sa + sg * sg - sd * sd)
+    beta = np.where(sg > 0.0, cov / np.maximum(sg * sg, 1e-300), 0.0)
+    price = pa - beta * (pg - np.asarray(geo_exact, np.float64))
+    se = np.sqrt(np.maximum(sa * sa - beta * cov, 0.0))
+    return price, se, beta
 
 
 def asian_weights(req: Request, chain: int):

@@ -24,10 +24,7 @@ double now_s() {
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-uint32_t n_payoffs(uint32_t payoffs) {
-    return ((payoffs & HEXO_ASIAN) ? 1u : 0u) + ((payoffs & HEXO_EUROPEAN) ? 1u : 0u) +
-           ((payoffs & HEXO_ASIAN_GEO_CV) ? 1u : 0u);
-}
+uint32_t n_payoffs(uint32_t payoffs) { return (uint32_t)__builtin_popcount(payoffs & 15u); }
 
 uint64_t n_opts_of(const hexo_request* r) {
     uint64_t n = 0;
@@ -113,8 +110,8 @@ std::string validate(const hexo_request* r) {
     if (!r->strikes) return "strikes must be non-NULL";
     for (uint64_t i = 0; i < n_opts; ++i)
         if (!fin(r->strikes[i])) return "strikes must be finite";
-    if (r->payoffs == 0 || (r->payoffs & ~7u))
-        return "payoffs must be a non-empty subset of ASIAN|EUROPEAN|ASIAN_GEO_CV";
+    if (r->payoffs == 0 || (r->payoffs & ~15u))
+        return "payoffs must be a non-empty subset of ASIAN|EUROPEAN|ASIAN_GEO_CV|GEO_ASIAN";
     if (r->flags & ~15u) return "unknown flag bits";
     if (r->precision > 2u) return "precision must be HEXO_FP64, HEXO_FP32 or HEXO_FP32_COMPENSATED";
     const uint64_t n_tiles = (r->n_paths + kTile - 1) / kTile;
@@ -123,7 +120,7 @@ std::string validate(const hexo_request* r) {
     uint64_t total = 0;
     if (!build_schedule(r->expiries, r->n_expiries, r->n_strikes, r->steps, segs, total))
         return "time grid could not be built";
-    if (r->payoffs & HEXO_ASIAN_GEO_CV) {
+    if (r->payoffs & (HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN)) {
         GeoPlan g;
         std::string why;
         if (!plan_geo(r, segs, g, why)) return why;
@@ -269,7 +266,8 @@ bool plan_request(const hexo_request* r, ReqPlan& p) {
     p.n_groups = (p.n_tiles + kGroupTiles - 1) / kGroupTiles;
     p.cls = r->precision == HEXO_FP64
                 ? 0u
-                : 1u + ((r->flags & HEXO_MARTINGALE) ? 1u : 0u) + ((r->payoffs & HEXO_ASIAN_GEO_CV) ? 2u : 0u);
+                : 1u + ((r->flags & HEXO_MARTINGALE) ? 1u : 0u) +
+                      ((r->payoffs & (HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN)) ? 2u : 0u);
     if (!plan_geo(r, p.segs, p.geo, p.geo_err) && p.geo_err.empty()) p.geo_err = "geometric control variate";
     p.n_fins = 0;
     for (auto& s : p.segs) p.n_fins += (uint32_t)s.fins.size();

@@ -98,8 +98,9 @@ int64_t functional_weights(const std::vector<HostSeg>& segs, uint32_t flags, uin
 
 bool plan_geo(const hexo_request* r, const std::vector<HostSeg>& segs, GeoPlan& g, std::string& why) {
     g = GeoPlan();
+    // W for both geometric payoffs; exact prices for the control variate only
     const bool cv = (r->payoffs & HEXO_ASIAN_GEO_CV) != 0;
-    if (!cv) {
+    if (!(r->payoffs & (HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN))) {
         for (const HostSeg& S : segs) g.fin_W.insert(g.fin_W.end(), S.fins.size(), 1.0);
         return true;
     }
@@ -138,6 +139,7 @@ bool plan_geo(const hexo_request* r, const std::vector<HostSeg>& segs, GeoPlan& 
             mu += (t[j] - t[j - 1]) * (rd - 0.5 * ev) * suffix;
             var += (t[j] - t[j - 1]) * ev * suffix * suffix;
         }
+        if (!cv) return true;
         for (int64_t j = 0; j <= n; ++j) {
             g.pts.push_back(t[j]);
             g.pts.push_back(w[j] / W);

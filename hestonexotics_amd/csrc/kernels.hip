@@ -28,6 +28,7 @@ namespace {
 
 constexpr uint32_t kEuropean = 2u;
 constexpr uint32_t kAsianGeoCV = 4u;
+constexpr uint32_t kGeoAsian = 8u;
 constexpr uint32_t kTrapezoid = 4u;
 constexpr uint32_t kMartingale = 8u;
 constexpr uint32_t kPrecFP64 = 0u;
@@ -136,7 +137,8 @@ struct TileSums<float> {  // compensated FP32 sums (HEXO_FP32_COMPENSATED)
 };
 
 // Payoff kind of output block `pay` of a request: the pay-th set bit of its payoffs (outputs are
-// payoff-major in bit order: Asian, European, Asian with the geometric control variate).
+// payoff-major in bit order: Asian, European, Asian with the geometric control variate, the
+// geometric call).
 __device__ __forceinline__ uint32_t payoff_kind(uint32_t payoffs, uint32_t pay) {
     uint32_t b = payoffs;
     for (uint32_t i = 0; i < pay; ++i) b &= b - 1u;
@@ -151,12 +153,14 @@ __device__ __forceinline__ float payoff_of(float2 v, uint32_t kind, float K, boo
     return __builtin_fmaxf((kind == kEuropean ? v.y : v.x) - K, 0.0f);
 }
 __device__ __forceinline__ float payoff_of(float4 v, uint32_t kind, float K, bool) {
+    const float pg = __builtin_fmaxf(v.z - K, 0.0f);
     const float pa = __builtin_fmaxf((kind == kEuropean ? v.y : v.x) - K, 0.0f);
-    return kind == kAsianGeoCV ? pa - __builtin_fmaxf(v.z - K, 0.0f) : pa;
+    return kind == kGeoAsian ? pg : kind == kAsianGeoCV ? pa - pg : pa;
 }
 __device__ __forceinline__ double payoff_of(double4 v, uint32_t kind, double K, bool valid) {
+    const double pg = __builtin_fmax(v.z - K, 0.0);
     const double pa = __builtin_fmax((kind == kEuropean ? v.y : v.x) - K, 0.0);
-    const double pay = kind == kAsianGeoCV ? pa - __builtin_fmax(v.z - K, 0.0) : pa;
+    const double pay = kind == kGeoAsian ? pg : kind == kAsianGeoCV ? pa - pg : pa;
     return valid ? pay : 0.0;
 }
 
@@ -608,7 +612,7 @@ __device__ __forceinline__ bool central_f64(uint32_t k) { return __builtin_fabs(
 template <bool DEBUG, bool COUNT>
 __device__ __forceinline__ void path_fp64(const PathArgs& a, uint32_t ri, uint32_t tile, int lane, double4* vals) {
     const DevReq& R = a.reqs[ri];
-    const bool geo = (R.payoffs & kAsianGeoCV) != 0;  // wave-uniform
+    const bool geo = (R.payoffs & (kAsianGeoCV | kGeoAsian)) != 0;  // wave-uniform
     double accY = 0.0, yp = 0.0;  // the Asian accumulation on y = ln(X/S0); y before the last step
     const uint64_t pl = (uint64_t)tile * kTile + (uint64_t)lane;
     const bool valid = pl < R.n_paths;

@@ -49,6 +49,10 @@ typedef enum hexo_status {
  * estimator up to the scheme's discretisation of G; far smaller standard error. Needs every
  * expiry's weight sum W >= 0.25 (at least 2 steps). Outputs are payoff-major in bit order. */
 #define HEXO_ASIAN_GEO_CV 4u
+/* The geometric-average call of the same functional, max(G - K, 0), by plain Monte Carlo. With
+ * HEXO_ASIAN, HEXO_ASIAN_GEO_CV and hexo_geo_asian_price it gives the variances and covariance for
+ * the control variate with the regression coefficient instead of 1 (never worse than plain). */
+#define HEXO_GEO_ASIAN 8u
 
 /* flags: documented extensions; 0 reproduces the reference exactly */
 #define HEXO_DRIFT 1u     /* add r*delta to the log-price step (the reference has no drift term) */
@@ -88,7 +92,7 @@ typedef struct hexo_request {
     const double* expiries;
     const uint32_t* n_strikes;
     const double* strikes;
-    uint32_t payoffs;   /* HEXO_ASIAN | HEXO_EUROPEAN | HEXO_ASIAN_GEO_CV */
+    uint32_t payoffs;   /* HEXO_ASIAN | HEXO_EUROPEAN | HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN */
     uint32_t flags;     /* HEXO_DRIFT | HEXO_DISCOUNT | HEXO_TRAPEZOID | HEXO_MARTINGALE */
     uint32_t precision; /* HEXO_FP64 | HEXO_FP32 | HEXO_FP32_COMPENSATED */
     uint32_t reserved;  /* must be 0 */
@@ -123,7 +127,8 @@ const char* hexo_version(void);
  * model's domain; returns HEXO_OK or HEXO_ERR_INVALID. */
 int hexo_validate_request(const hexo_request* req);
 /* Number of outputs of a request: n_payoffs * sum(n_strikes). Outputs are payoff-major
- * (Asian, European, control-variate Asian blocks, those requested), options in chain-major order. 0 if the request is invalid. */
+ * (Asian, European, control-variate Asian, geometric blocks, those requested), options in chain-major
+ * order. 0 if the request is invalid. */
 uint64_t hexo_request_n_outputs(const hexo_request* req);
 /* The reference's time grid (src/HSimulation.tpp:30-45 with src/inc/AsianContract.h:29-38),
  * replayed in double: fin_step[c] = 1-based step at which expiry c is finalised,

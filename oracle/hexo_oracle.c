@@ -437,7 +437,7 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
     uint32_t blk[4];
     uint64_t have = ~(uint64_t)0;
     const int want_asian = (payoffs & ORC_ASIAN) != 0, want_euro = (payoffs & ORC_EUROPEAN) != 0;
-    const int want_cv = (payoffs & ORC_ASIAN_GEO_CV) != 0;
+    const int want_cv = (payoffs & ORC_ASIAN_GEO_CV) != 0, want_geo = (payoffs & ORC_GEO_ASIAN) != 0;
     double logXp = logX;
     uint64_t nq = 0, ne = 0;
 
@@ -479,7 +479,7 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
             /* geometric counterpart: the same functional on y = ln(X/S0), normalised by its
              * weight sum W; G = W S0 exp(L / W) */
             double geo = 0.0;
-            if (want_cv) {
+            if (want_cv || want_geo) {
                 const double y = logX - logS0, yp = logXp - logS0, wf = (T[c] - tp) / dt;
                 double L, W;
                 if (flags & ORC_TRAPEZOID) {
@@ -506,10 +506,16 @@ static void simulate_path(const double params[5], double S0, double r, uint32_t 
                     acc_add(&sumsq[o], pay * pay);
                     o += n_opts;
                 }
+                const double pg = geo - K > 0.0 ? geo - K : 0.0;
                 if (want_cv) {
-                    const double pa = avg - K > 0.0 ? avg - K : 0.0, pg = geo - K > 0.0 ? geo - K : 0.0;
+                    const double pa = avg - K > 0.0 ? avg - K : 0.0;
                     acc_add(&sums[o], pa - pg);
                     acc_add(&sumsq[o], (pa - pg) * (pa - pg));
+                    o += n_opts;
+                }
+                if (want_geo) {
+                    acc_add(&sums[o], pg);
+                    acc_add(&sumsq[o], pg * pg);
                 }
             }
             priced += n_strikes[c];
@@ -537,9 +543,7 @@ int orc_price(const double params[5], double S0, double r, uint32_t flags, uint3
               uint64_t path_offset, uint64_t seed, uint32_t steps, uint32_t n_exp, const double* T,
               const uint32_t* n_strikes, const double* strikes, double* prices, double* std_errs, double* finals,
               uint64_t* counts, int n_threads) {
-    if (n_paths == 0 || steps == 0 || n_exp == 0 || !(payoffs & (ORC_ASIAN | ORC_EUROPEAN | ORC_ASIAN_GEO_CV)) ||
-        (payoffs & ~7u))
-        return -1;
+    if (n_paths == 0 || steps == 0 || n_exp == 0 || !(payoffs & 15u) || (payoffs & ~15u)) return -1;
     for (uint32_t c = 0; c < n_exp; ++c)
         if (!(T[c] > 0.0) || (c > 0 && !(T[c] > T[c - 1]))) return -2;
     uint32_t* opt_off = (uint32_t*)malloc(sizeof(uint32_t) * (n_exp + 1));
@@ -554,7 +558,9 @@ int orc_price(const double params[5], double S0, double r, uint32_t flags, uint3
         return -1;
     }
     const int n_pay = ((payoffs & ORC_ASIAN) ? 1 : 0) + ((payoffs & ORC_EUROPEAN) ? 1 : 0) +
-                      ((payoffs & ORC_ASIAN_GEO_CV) ? 1 : 0);
+                      ((payoffs & ORC_ASIAN_GEO_CV) ? 1 : 0) + ((payoffs & ORC_GEO_ASIAN) ? 1 : 0);
+    /* block of the control-variate outputs: after the Asian and European blocks */
+    const uint64_t cv_block = ((payoffs & ORC_ASIAN) ? 1 : 0) + ((payoffs & ORC_EUROPEAN) ? 1 : 0);
     double* geo_exact = NULL;
     if (payoffs & ORC_ASIAN_GEO_CV) {
         geo_exact = (double*)malloc(sizeof(double) * n_opts);
@@ -617,8 +623,7 @@ int orc_price(const double params[5], double S0, double r, uint32_t flags, uint3
         const double mean = acc_val(&s) / n;
         double var = n > 1 ? (acc_val(&ss) - n * mean * mean) / (n - 1.0) : 0.0;
         if (var < 0) var = 0;
-        /* the control-variate outputs are the last n_opts */
-        const double exact = (geo_exact && o >= n_out - n_opts) ? geo_exact[opt] : 0.0;
+        const double exact = (geo_exact && o / n_opts == cv_block) ? geo_exact[opt] : 0.0;
         prices[o] = disc * (mean + exact);
         if (std_errs) std_errs[o] = disc * sqrt(var / n);
     }

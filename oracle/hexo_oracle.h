@@ -41,6 +41,7 @@ extern "C" {
 #define ORC_TRAPEZOID 4u /* corrected trapezoid Asian average (extension) */
 #define ORC_MARTINGALE 8u /* Andersen's martingale-corrected K0 (extension) */
 #define ORC_ASIAN_GEO_CV 4u /* payoff: Asian call with the geometric-average control variate (extension) */
+#define ORC_GEO_ASIAN 8u    /* payoff: the geometric-average call itself, plain Monte Carlo (extension) */
 
 /* ---- Philox4x32-10 ---- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
@@ -78,8 +79,8 @@ int64_t orc_time_grid(const double* T, uint32_t n_exp, const uint32_t* n_strikes
 
 /* ---- Monte-Carlo price (reference semantics, FP64) ----
  * Prices every option of every chain. Outputs are payoff-major: for each payoff present in
- * `payoffs` (Asian, European, then the Asian with the geometric control variate), n_opts
- * entries in chain-major order. The control-variate price is mean(A payoff - G payoff) plus the
+ * `payoffs` (Asian, European, the Asian with the geometric control variate, the geometric call),
+ * n_opts entries in chain-major order. The control-variate price is mean(A payoff - G payoff) plus the
  * exact geometric price (orc_geo_asian_price), times the discount factor.
  * finals (nullable): [n_paths][n_exp][2] = (Asian average A_c, interpolated X_T) per path.
  * counts (nullable): [2] = (quadratic-branch steps, exponential-branch steps).

@@ -16,7 +16,7 @@ import platform
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ASIAN, EUROPEAN, ASIAN_GEO_CV = 1, 2, 4
+ASIAN, EUROPEAN, ASIAN_GEO_CV, GEO_ASIAN = 1, 2, 4, 8
 DRIFT, DISCOUNT, TRAPEZOID, MARTINGALE = 1, 2, 4, 8
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -193,7 +193,7 @@ def price(params, S0: float, T, n_strikes, strikes, *, steps: int, n_paths: int,
     T = np.ascontiguousarray(np.atleast_1d(T), np.float64)
     ns = np.ascontiguousarray(np.atleast_1d(n_strikes), np.uint32)
     K = np.ascontiguousarray(np.atleast_1d(strikes), np.float64)
-    n_pay = (1 if payoffs & ASIAN else 0) + (1 if payoffs & EUROPEAN else 0) + (1 if payoffs & ASIAN_GEO_CV else 0)
+    n_pay = bin(payoffs & 15).count("1")
     n_out = int(ns.sum()) * n_pay
     prices = np.zeros(n_out, np.float64)
     ses = np.zeros(n_out, np.float64)

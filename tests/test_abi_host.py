@@ -53,7 +53,7 @@ MUTATIONS = {
     "strike_nan": lambda r: setattr(r, "strikes", [[float("inf")]]),
     "no_strikes": lambda r: setattr(r, "strikes", [[]]),
     "payoffs": lambda r: setattr(r, "payoffs", 0),
-    "payoff_bits": lambda r: setattr(r, "payoffs", 8),
+    "payoff_bits": lambda r: setattr(r, "payoffs", 16),
     "flags": lambda r: setattr(r, "flags", 64),
     "precision": lambda r: setattr(r, "precision", 7),
     "path_offset_unaligned": lambda r: setattr(r, "path_offset", 100),

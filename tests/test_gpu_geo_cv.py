@@ -140,6 +140,28 @@ def test_fp32_control_variate_close_to_fp64(ctx, name):
             assert rel.max() < 1e-4
 
 
+@pytest.mark.parametrize("name", sorted(geo_requests()))
+def test_geometric_block_and_regression_estimator(ctx, oracle, name):
+    """The plain geometric-call block (GEO_ASIAN) equals the oracle's on the same draws (FP64
+    kernel); with it the regression-coefficient control variate is never worse than the plain
+    estimator or the coefficient-1 one (FP32 kernel)."""
+    say("gpu geo block", name)
+    req = cv(geo_requests()[name], H.FP64, H.ASIAN | H.ASIAN_GEO_CV | H.GEO_ASIAN)
+    p, s = ctx.price(req)
+    o = oracle_price(oracle, req)
+    n = req.n_opts
+    o_fma = oracle_price(oracle, req, kind="fma")
+    tol = max(1e-12 * req.S0, 2.0 * np.abs(o_fma["prices"][2 * n:] - o["prices"][2 * n:]).max())
+    np.testing.assert_allclose(p[2 * n:], o["prices"][2 * n:], rtol=1e-12, atol=tol)
+    req.precision = H.FP32
+    p, s = ctx.price(req)
+    exact = ctx.geo_asian_price(req)
+    price, se, beta = H.cv_optimal((p[:n], s[:n]), (p[2 * n:], s[2 * n:]), (p[n:2 * n], s[n:2 * n]), exact)
+    say(f"  beta {np.round(beta, 3)}, SE plain {s[:n]}, cv {s[n:2 * n]}, optimal {se}")
+    assert np.all(se <= s[:n] * (1 + 1e-9)) and np.all(se <= s[n:2 * n] * (1 + 1e-9))
+    assert np.all(np.abs(price - p[:n]) < 4 * s[:n] + 1e-12)
+
+
 def test_mixed_batch_equals_single(ctx):
     say("gpu geo cv mixed batch == single")
     base = geo_requests()["c1_like"]
@@ -150,6 +172,8 @@ def test_mixed_batch_equals_single(ctx):
         cv(base, H.FP32, H.ASIAN | H.EUROPEAN | H.ASIAN_GEO_CV),
         cv(geo_requests()["multi_expiry"], H.FP32_COMPENSATED, H.EUROPEAN | H.ASIAN_GEO_CV),
         cv(geo_requests()["stiff_c5"], H.FP32, H.ASIAN_GEO_CV, H.MARTINGALE),
+        cv(geo_requests()["peaked"], H.FP32, H.GEO_ASIAN),
+        cv(geo_requests()["positive_rho"], H.FP64, H.EUROPEAN | H.GEO_ASIAN),
     ]
     together = ctx.price_batch(variants)
     for r, (p, s) in zip(variants, together):

@@ -129,3 +129,24 @@ def test_control_variate_output_order_and_exact_part():
     disc = np.exp(-0.02 * np.array([0.5, 0.5, 1.0]))
     diff = alone["prices"] - disc * geo
     assert np.all(np.abs(diff) < 1.0)   # a - g is small per path: mean of the difference
+
+
+def test_regression_coefficient_estimator_is_never_worse():
+    """With the geometric call's own block the control variate can use beta = Cov(A, G)/Var(G):
+    its standard error is at most the plain and the beta = 1 ones, and where the reference's final
+    term decouples A and G (T = 0.1 on 5 steps) beta falls well below 1."""
+    import hestonexotics_amd as H
+    params, S0 = [0.04, 0.04, -0.7, 2.0, 0.3], 100.0
+    for T, N in ((0.1, 5), (1.0, 52)):
+        K = [80.0, 100.0, 120.0]
+        res = O.price(params, S0, [T], [3], K, steps=N, n_paths=100_000, seed=5, r=0.02, flags=O.DRIFT,
+                      payoffs=O.ASIAN | O.ASIAN_GEO_CV | O.GEO_ASIAN)
+        p, s = res["prices"], res["std_errs"]
+        exact = O.geo_asian_price(params, S0, [T], [3], K, steps=N, r=0.02)
+        # the geometric block is a plain estimate of the exact price
+        assert np.all(np.abs(p[6:] - exact) < 4 * s[6:] + 1e-3)
+        price, se, beta = H.cv_optimal((p[:3], s[:3]), (p[6:], s[6:]), (p[3:6], s[3:6]), exact)
+        assert np.all(se <= s[:3] * (1 + 1e-9)) and np.all(se <= s[3:6] * (1 + 1e-9)), (se, s)
+        assert np.all(np.abs(price - p[:3]) < 4 * s[:3])
+        if T == 0.1:
+            assert beta[1] < 0.9, beta
