@@ -242,6 +242,13 @@ def test_graph_replays_match_eager(ctx):
     gives the eager results every time."""
     say("gpu graph replays")
     reqs = W.c2(n_paths=20_000, maturities=[0.25, 1.0], n_strikes=8)
+    # every kernel class in the captured sequence: martingale-corrected and control-variate
+    # requests (whose exact geometric prices are computed once, before capture) beside plain ones
+    extra = W.c2(n_paths=20_000, maturities=[0.5, 2.0], n_strikes=4)
+    extra[0].flags |= H.MARTINGALE
+    extra[1].payoffs |= H.ASIAN_GEO_CV | H.GEO_ASIAN
+    extra[1].flags |= H.MARTINGALE
+    reqs += extra + [W.c2(n_paths=3_000, maturities=[1.0], n_strikes=4, precision=H.FP64)[0]]
     other = small_requests()["c1_like"]
     other.precision = H.FP32
     eager = ctx.price_batch(reqs)

@@ -20,6 +20,15 @@ def small_trace(n=40, seed=3):
     reqs, arr = W.c4(rate=400.0, duration_s=n / 400.0 * 1.2, seed=seed, path_scale=0.005)
     reqs = reqs[:n]
     arr = arr[:n]
+    # every kernel class in the stream: some requests with the martingale correction, some with
+    # the control-variate and geometric blocks, one in FP64
+    for i, r in enumerate(reqs):
+        if i % 5 == 1:
+            r.flags |= H.MARTINGALE
+        if i % 7 == 2 and r.payoffs & H.ASIAN:
+            r.payoffs |= H.ASIAN_GEO_CV | H.GEO_ASIAN
+    if len(reqs) > 3:
+        reqs[3].precision = H.FP64
     return reqs, arr
 
 
