@@ -294,51 +294,30 @@ struct StepCount {
     uint32_t quad = 0, exp = 0, central = 0, tail = 0;
 };
 
-// One QE step for the PPT paths of a lane (Andersen's QE scheme as in src/HSimulation.tpp:52-86):
-// variance by the quadratic branch a (b + Z_V)^2 when psi = s2/m^2 < 1.5, else the exponential
-// branch; then the central log-price step. Y is log2(X/S0); X = 2^Y; P accumulates X for the
-// Asian sum. Z_V (quadratic lanes) and Z_X come from AS241 with the tails gathered per wave.
-// MART: Andersen's martingale-corrected step (the DevSeg form), falling back per path and step to
-// the plain step where E[exp(A V')] does not exist. GEO: also keeps Y before the step (Yp) and the
-// run's sum of Y (PY), for the geometric control variate.
-template <int PPT, bool COUNT, bool MART, bool GEO>
-__device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], float (&X)[PPT], float (&Xp)[PPT],
-                                             float (&P)[PPT], float (&Yp)[PPT], float (&PY)[PPT],
-                                             const uint32_t (&wv)[PPT], const uint32_t (&wx)[PPT],
-                                             const SegF& c, uint32_t* q, int lane, const bool (&valid)[PPT],
-                                             StepCount& cnt) {
-    float m[PPT], s2[PPT], m2[PPT];
-    bool quad[PPT];
-    uint32_t w[2 * PPT];
-    bool t[2 * PPT];
-    uint64_t mk[2 * PPT];
-    float z[2 * PPT];
+// The step's conditional moments from V (src/HSimulation.tpp:58-62): m, s2, m^2 and the branch
+// (quadratic when psi = s2/m^2 < 1.5).
+template <int PPT>
+__device__ __forceinline__ void qe_moments_fp32(const float (&V)[PPT], const SegF& c, float (&m)[PPT],
+                                                float (&s2)[PPT], float (&m2)[PPT], bool (&quad)[PPT]) {
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         m[j] = __builtin_fmaf(c.omd, c.th - V[j], V[j]);
         s2[j] = __builtin_fmaf(V[j], c.c1, c.c2);
         m2[j] = m[j] * m[j];
         quad[j] = s2[j] < 1.5f * m2[j];
-        bool cv, cx;
-        z[2 * j] = normal_central_f32(wv[j], cv);
-        z[2 * j + 1] = normal_central_f32(wx[j], cx);
-        w[2 * j] = wv[j];
-        w[2 * j + 1] = wx[j];
-        t[2 * j] = quad[j] && !cv;
-        t[2 * j + 1] = !cx;
-        // ballots of the comparisons themselves (a ballot of the combined bool costs two VALU ops)
-        mk[2 * j] = __builtin_amdgcn_ballot_w64(quad[j]) & __builtin_amdgcn_ballot_w64(!cv);
-        mk[2 * j + 1] = __builtin_amdgcn_ballot_w64(!cx);
-        if constexpr (COUNT) {
-            if (valid[j]) {
-                cnt.quad += quad[j] ? 1u : 0u;
-                cnt.exp += quad[j] ? 0u : 1u;
-                cnt.central += (quad[j] && cv ? 1u : 0u) + (cx ? 1u : 0u);
-                cnt.tail += (t[2 * j] ? 1u : 0u) + (t[2 * j + 1] ? 1u : 0u);
-            }
-        }
     }
-    tail_normals<2 * PPT>(w, t, mk, z, q, lane);
+}
+
+// The update of one step given its moments and normals: the variance by the quadratic branch
+// a (b + Z_V)^2 or the exponential branch, then the log-price step (MART: Andersen's
+// martingale-corrected form, falling back per path and step to the plain step where
+// E[exp(A V')] does not exist; GEO: also Y before the step and the run's sum of Y).
+template <int PPT, bool MART, bool GEO>
+__device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT], float (&X)[PPT], float (&Xp)[PPT],
+                                               float (&P)[PPT], float (&Yp)[PPT], float (&PY)[PPT],
+                                               const uint32_t (&wv)[PPT], const float (&zv)[PPT],
+                                               const float (&zx)[PPT], const float (&m)[PPT], const float (&s2)[PPT],
+                                               const float (&m2)[PPT], const bool (&quad)[PPT], const SegF& c) {
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         float Vn;
@@ -356,7 +335,7 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
             // mean E[V'] = a (b^2 + 1) = m holds up to unbiased rounding: a biased mean moves the
             // variance's mean-reversion level by (relative bias) / (kappa dt)
             const float aq = m[j] * rcp_refined(__builtin_fmaf(b, b, 1.0f));
-            const float bz = b + z[2 * j];
+            const float bz = b + zv[j];
             Vn = aq * bz * bz;
             if constexpr (MART) {
                 // x = 2 A a < 1: A m - ln E[exp(A V')] = -(f(-x) + x^2 b^2 / (1 - x)) / 2
@@ -408,13 +387,70 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
         }
         const float sq = __builtin_amdgcn_sqrtf(c.k3 * sum);
         if constexpr (GEO) Yp[j] = Y[j];
-        Y[j] = __builtin_fmaf(sq, z[2 * j + 1], Y[j] + inc);
+        Y[j] = __builtin_fmaf(sq, zx[j], Y[j] + inc);
         Xp[j] = X[j];
         X[j] = __builtin_amdgcn_exp2f(Y[j]);
         P[j] += X[j];
         if constexpr (GEO) PY[j] += Y[j];
         V[j] = Vn;
     }
+}
+
+
+// Appends the normals of one step for the tail queue: slots s0 (Z_V) and s0 + 1 (Z_X) of path j.
+// need_v: Z_V is needed for sure (its step's branch is known to be quadratic) or possibly.
+template <int N>
+__device__ __forceinline__ void queue_normals(uint32_t wv, uint32_t wx, bool need_v, int s0, float (&z)[N],
+                                              uint32_t (&w)[N], bool (&t)[N], uint64_t (&mk)[N], bool& cv,
+                                              bool& cx) {
+    z[s0] = normal_central_f32(wv, cv);
+    z[s0 + 1] = normal_central_f32(wx, cx);
+    w[s0] = wv;
+    w[s0 + 1] = wx;
+    t[s0] = need_v && !cv;
+    t[s0 + 1] = !cx;
+    // ballots of the comparisons themselves (a ballot of the combined bool costs two VALU ops)
+    mk[s0] = __builtin_amdgcn_ballot_w64(need_v) & __builtin_amdgcn_ballot_w64(!cv);
+    mk[s0 + 1] = __builtin_amdgcn_ballot_w64(!cx);
+}
+
+__device__ __forceinline__ void count_step(StepCount& cnt, bool valid, bool quad, bool cv, bool cx) {
+    if (!valid) return;
+    cnt.quad += quad ? 1u : 0u;
+    cnt.exp += quad ? 0u : 1u;
+    cnt.central += (quad && cv ? 1u : 0u) + (cx ? 1u : 0u);
+    cnt.tail += (quad && !cv ? 1u : 0u) + (cx ? 0u : 1u);
+}
+
+// One QE step for the PPT paths of a lane (Andersen's QE scheme as in src/HSimulation.tpp:52-86):
+// the normals Z_V (quadratic lanes) and Z_X from AS241 with the tails gathered per wave, then the
+// update. Y is log2(X/S0); X = 2^Y; P accumulates X for the Asian sum.
+template <int PPT, bool COUNT, bool MART, bool GEO>
+__device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], float (&X)[PPT], float (&Xp)[PPT],
+                                             float (&P)[PPT], float (&Yp)[PPT], float (&PY)[PPT],
+                                             const uint32_t (&wv)[PPT], const uint32_t (&wx)[PPT],
+                                             const SegF& c, uint32_t* q, int lane, const bool (&valid)[PPT],
+                                             StepCount& cnt) {
+    float m[PPT], s2[PPT], m2[PPT], zv[PPT], zx[PPT];
+    bool quad[PPT];
+    uint32_t w[2 * PPT];
+    bool t[2 * PPT];
+    uint64_t mk[2 * PPT];
+    float z[2 * PPT];
+    qe_moments_fp32<PPT>(V, c, m, s2, m2, quad);
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        bool cv, cx;
+        queue_normals<2 * PPT>(wv[j], wx[j], quad[j], 2 * j, z, w, t, mk, cv, cx);
+        if constexpr (COUNT) count_step(cnt, valid[j], quad[j], cv, cx);
+    }
+    tail_normals<2 * PPT>(w, t, mk, z, q, lane);
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        zv[j] = z[2 * j];
+        zx[j] = z[2 * j + 1];
+    }
+    qe_update_fp32<PPT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, wv, zv, zx, m, s2, m2, quad, c);
 }
 
 template <int PPT>
