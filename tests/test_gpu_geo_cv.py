@@ -162,6 +162,26 @@ def test_geometric_block_and_regression_estimator(ctx, oracle, name):
     assert np.all(np.abs(price - p[:n]) < 4 * s[:n] + 1e-12)
 
 
+def test_exact_prices_over_the_uneven_request_distribution(ctx, oracle):
+    """The 512 uneven requests' distribution (kappa up to 20, sigma up to 1, T from 0.1 to 5 on
+    252 steps a year) with the control variate, in one batch: every exact geometric price is
+    finite, and a sample of them equals the oracle's to 1e-9 S0."""
+    say("gpu geo exact over the uneven request distribution")
+    reqs = W.c3(precision=H.FP32, path_scale=1e-4, payoffs=H.ASIAN_GEO_CV)
+    out = ctx.price_batch(reqs)
+    assert all(np.all(np.isfinite(p)) for p, _ in out)
+    rng = np.random.default_rng(1)
+    worst = 0.0
+    for i in rng.choice(len(reqs), 8, replace=False):
+        q = reqs[int(i)]
+        g = ctx.geo_asian_price(q)
+        o = oracle.geo_asian_price(q.params, q.S0, q.expiries, q.n_strikes(), q.flat_strikes(), steps=q.steps,
+                                   r=q.r, flags=q.flags) * discounts(q)
+        worst = max(worst, float(np.abs(g - o).max() / q.S0))
+    say(f"  max |gpu - oracle| / S0 over the sample: {worst:.2e}")
+    assert worst < 1e-9
+
+
 def test_mixed_batch_equals_single(ctx):
     say("gpu geo cv mixed batch == single")
     base = geo_requests()["c1_like"]
