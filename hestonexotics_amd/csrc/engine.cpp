@@ -214,7 +214,7 @@ struct Layout {
     size_t reqs, segs, fins, opts, order[kPathClasses], item_end[kPathClasses], group_end, out_end;  // uploaded
     size_t upload_end;
     size_t geo_items, geo_pts, gl, geo_wave_end;                                      // uploaded
-    size_t counters, counts, partials, groups, prices, ses, finals, geo, geo_panel, geo_pmax;  // device only
+    size_t counters, counts, partials, groups, prices, ses, finals, geo, geo_panel, geo_pmax, geo_state;  // device
     size_t total;
 };
 
@@ -253,6 +253,7 @@ Layout make_layout(uint32_t n_req, uint64_t n_seg, uint64_t n_fin, uint64_t n_op
     L.geo = take(sizeof(double) * std::max<uint64_t>(n_out, 1));
     L.geo_panel = take(64 * sizeof(double) * std::max<uint64_t>(n_geo_waves, 1));
     L.geo_pmax = take(sizeof(double) * std::max<uint64_t>(n_geo_waves, 1));
+    L.geo_state = take(sizeof(double) * std::max<size_t>(geo_state_doubles((uint32_t)n_geo_items), 1));
     L.total = o;
     return L;
 }
@@ -281,7 +282,7 @@ size_t request_device_bytes(const hexo_request* r) {
     if (!plan_request(r, p)) return 0;
     return sizeof(DevReq) + p.segs.size() * sizeof(DevSeg) + p.n_fins * sizeof(DevFin) + p.n_opts * sizeof(DevOpt) +
            32 + 16 + (size_t)16 * p.n_tiles * p.n_out + (size_t)16 * p.n_groups * p.n_out + 24 * p.n_out +
-           p.geo.items.size() * (sizeof(DevGeo) + 4 + (kGeoMaxP0 + 1) * 520) + p.geo.pts.size() * sizeof(double);
+           p.geo.items.size() * (sizeof(DevGeo) + 4 + (kGeoMaxP0 + 1) * 520 + 8 * 67) + p.geo.pts.size() * sizeof(double);
 }
 
 size_t batch_device_bytes(const hexo_request* reqs, uint32_t n) {
@@ -596,6 +597,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     ga.wave_end = reinterpret_cast<const uint32_t*>(D + L.geo_wave_end);
     ga.panel = reinterpret_cast<double*>(D + L.geo_panel);
     ga.panel_max = reinterpret_cast<double*>(D + L.geo_pmax);
+    ga.state = reinterpret_cast<double*>(D + L.geo_state);
     ra.req_of = nullptr;
     ra.group_end = reinterpret_cast<const uint64_t*>(D + L.group_end);
     ra.out_end = reinterpret_cast<const uint64_t*>(D + L.out_end);

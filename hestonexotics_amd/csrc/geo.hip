@@ -13,13 +13,15 @@
 // k = ln(K / c), by 64-node Gauss-Legendre panels of width h (host-chosen from the scale of L and
 // the strikes) until the characteristic function has decayed below 1e-15 on a whole panel.
 //
-// Work items are (request, expiry, block of <= 64 strikes). Pass 1 evaluates the first p0 panels
-// of every item at once, one wavefront per panel (lane q: node q, a recursion over the expiry's
-// grid, a few hundred to a few thousand steps; lane i: strike i's panel sum over the 64 nodes in
-// node order), plus E[e^L]; p0 is the host's estimate of where the characteristic function has
-// decayed. Pass 2 gives each item one wavefront: lane i adds strike i's panel sums in panel order
-// and evaluates further panels itself if the first p0 did not reach the decay; every price is
-// therefore the same fixed-order sum whatever p0 is. Runs once per batch, before the path kernels.
+// Work items are (request, expiry, block of <= 64 strikes), in up to three rounds. Pass 1 of a
+// round evaluates the next p0 panels of every unconverged item at once, one wavefront per panel
+// (lane q: node q, a recursion over the expiry's grid, a few hundred to a few thousand steps;
+// lane i: strike i's panel sum over the 64 nodes in node order), plus E[e^L] in the first round;
+// p0 is the host's estimate of where the characteristic function has decayed. Pass 2 gives each
+// item one wavefront: lane i adds strike i's panel sums in panel order, keeps them for the next
+// round if the decay was not reached, and in the last round evaluates further panels itself;
+// every price is therefore the same fixed-order sum whatever p0 is. Runs once per batch, before
+// the path kernels.
 #include <hip/hip_runtime.h>
 
 #include "cplx_device.hpp"
@@ -120,9 +122,15 @@ __device__ __forceinline__ void strike_of(const GeoArgs& a, const DevGeo& G, int
     k = has_k ? log(K / G.c) : 0.0;
 }
 
-// Pass 1: every wavefront evaluates one panel of one item (all panels of all items at once) and
-// stores each strike's panel sum; the item's last wavefront evaluates E[e^L] instead.
-__global__ __launch_bounds__(256) void k_geo_panels(GeoArgs a) {
+// Per-item state between rounds (kGeoState doubles): each strike's integral so far, E[e^L], the
+// panels summed so far and whether the sum has converged.
+constexpr int kGeoState = 64 + 3;
+constexpr int kGeoRounds = 3;  // parallel rounds of p0 panels before the serial continuation
+
+// Pass 1 of round r: every wavefront evaluates one panel of one unconverged item (round r covers
+// that item's panels done .. done + p0 - 1) and stores each strike's panel sum; in round 0 the
+// item's last wavefront evaluates E[e^L] instead.
+__global__ __launch_bounds__(256) void k_geo_panels(GeoArgs a, uint32_t round) {
     __shared__ double sh[kGeoWaves][3][64];
     const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
     const uint32_t wv = blockIdx.x * kGeoWaves + (uint32_t)w;
@@ -131,12 +139,19 @@ __global__ __launch_bounds__(256) void k_geo_panels(GeoArgs a) {
     const DevGeo& G = a.items[item];
     const DevReq& R = a.reqs[G.req];
     const double* pts = a.pts + 2 * (size_t)G.pts_begin;
-    const uint32_t p = wv - G.wave_begin;
-    if (p == G.p0) {
+    const uint32_t j = wv - G.wave_begin;
+    const double* st = a.state + (size_t)item * kGeoState;
+    uint32_t base = 0;
+    if (round > 0) {
+        if (j == G.p0 || st[66] != 0.0) return;  // E[e^L] is known; or the item has converged
+        base = (uint32_t)st[65];
+    } else if (j == G.p0) {
         const Cplx e1 = geo_cf(R, pts, G.n_pts, cx(1.0));
         if (lane == 0) a.panel_max[wv] = e1.re;
         return;
     }
+    const uint32_t p = base + j;
+    if (p >= G.max_panels) return;
     double K, k, mag;
     bool has_k;
     strike_of(a, G, lane, K, k, has_k);
@@ -144,10 +159,11 @@ __global__ __launch_bounds__(256) void k_geo_panels(GeoArgs a) {
     if (lane == 0) a.panel_max[wv] = mag;
 }
 
-// Pass 2: one wavefront per item; lane i adds strike i's panel sums in panel order up to the first
-// panel (after the first) whose |phi| stays below 1e-15, evaluating further panels itself if the
-// parallel pass did not reach it; the price is the same sum whatever p0 is.
-__global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a) {
+// Pass 2 of round r: one wavefront per item; lane i adds strike i's panel sums in panel order up
+// to the first panel (after the first) whose |phi| stays below 1e-15. An unconverged item keeps
+// its sums for the next round; in the last round the wavefront evaluates further panels itself.
+// The price is the same ordered sum whatever p0 and the number of rounds are.
+__global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a, uint32_t round) {
     __shared__ double sh[kGeoWaves][3][64];
     const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
     const uint32_t item = blockIdx.x * kGeoWaves + (uint32_t)w;
@@ -155,22 +171,36 @@ __global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a) {
     const DevGeo& G = a.items[item];
     const DevReq& R = a.reqs[G.req];
     const double* pts = a.pts + 2 * (size_t)G.pts_begin;
+    double* st = a.state + (size_t)item * kGeoState;
+    if (round > 0 && st[66] != 0.0) return;
     double K, k;
     bool has_k;
     strike_of(a, G, lane, K, k, has_k);
-    const double e1 = a.panel_max[G.wave_begin + G.p0];  // E[e^L]
-    double integ = 0.0;
+    const double e1 = round == 0 ? a.panel_max[G.wave_begin + G.p0] : st[64];  // E[e^L]
+    double integ = round == 0 ? 0.0 : st[lane];
+    const uint32_t base = round == 0 ? 0u : (uint32_t)st[65];
     bool converged = false;
-    uint32_t p = 0;
-    for (; p < G.p0 && !converged; ++p) {
-        integ += a.panel[(size_t)(G.wave_begin + p) * 64 + (uint32_t)lane];
-        converged = p > 0 && a.panel_max[G.wave_begin + p] < 1e-15;
+    uint32_t p = base;
+    for (uint32_t j = 0; j < G.p0 && p < G.max_panels && !converged; ++j, ++p) {
+        integ += a.panel[(size_t)(G.wave_begin + j) * 64 + (uint32_t)lane];
+        converged = p > 0 && a.panel_max[G.wave_begin + j] < 1e-15;
+    }
+    const bool last = round + 1 == kGeoRounds;
+    if (!converged && !last && p < G.max_panels) {
+        st[lane] = integ;
+        if (lane == 0) {
+            st[64] = e1;
+            st[65] = (double)p;
+            st[66] = 0.0;
+        }
+        return;
     }
     for (; p < G.max_panels && !converged; ++p) {
         double mag;
         integ += panel_sum(a, G, R, pts, p, lane, k, has_k, sh[w], &mag);
         converged = p > 0 && mag < 1e-15;  // wave-uniform
     }
+    if (lane == 0) st[66] = 1.0;  // later rounds skip the item
     if ((uint32_t)lane >= G.n_strikes) return;
     double price;
     if (!converged)
@@ -184,13 +214,19 @@ __global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a) {
 
 }  // namespace
 
+size_t geo_state_doubles(uint32_t n_items) { return (size_t)n_items * kGeoState; }
+
 hipError_t launch_geo(const GeoArgs& a, hipStream_t st) {
     if (a.n_items == 0) return hipSuccess;
-    k_geo_panels<<<(a.n_waves + kGeoWaves - 1) / kGeoWaves, 64 * kGeoWaves, 0, st>>>(a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    k_geo_sum<<<(a.n_items + kGeoWaves - 1) / kGeoWaves, 64 * kGeoWaves, 0, st>>>(a);
-    return hipGetLastError();
+    for (uint32_t r = 0; r < (uint32_t)kGeoRounds; ++r) {
+        k_geo_panels<<<(a.n_waves + kGeoWaves - 1) / kGeoWaves, 64 * kGeoWaves, 0, st>>>(a, r);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        k_geo_sum<<<(a.n_items + kGeoWaves - 1) / kGeoWaves, 64 * kGeoWaves, 0, st>>>(a, r);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace hexo

@@ -16,8 +16,10 @@ const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode, 
 size_t path_lds_bytes(uint32_t precision, uint32_t ppt, uint32_t block);
 hipError_t launch_paths(uint32_t precision, uint32_t ppt, PathMode mode, uint32_t grid, uint32_t block,
                         hipStream_t st, const PathArgs& a, bool mart = false, bool geo = false);
-// Exact prices of the geometric control variates (geo.hip): one wavefront per item.
+// Exact prices of the geometric control variates (geo.hip); GeoArgs.state holds
+// geo_state_doubles(n_items) doubles.
 hipError_t launch_geo(const GeoArgs& a, hipStream_t st);
+size_t geo_state_doubles(uint32_t n_items);
 hipError_t launch_reduce(const ReduceArgs& a, uint64_t n_group_threads, uint64_t n_out_threads, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);

@@ -125,6 +125,7 @@ struct GeoArgs {
     const uint32_t* wave_end;  // inclusive prefix over items of p0 + 1
     double* panel;          // per panel wave: each strike's panel sum (64 doubles)
     double* panel_max;      // per panel wave: max |phi| over its nodes; E[e^L] in the item's last wave
+    double* state;          // per item between rounds: strike sums, E[e^L], panels done, converged
 };
 
 struct ReduceArgs {
