@@ -349,7 +349,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by paths or "
-              f"requests across GPUs with one small reduction; see DESIGN.md).", file=sys.stderr)
+              f"requests across GPUs with one small reduction).", file=sys.stderr)
         sys.exit(2)
 
     import hestonexotics_amd as H

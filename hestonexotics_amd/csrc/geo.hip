@@ -128,7 +128,7 @@ constexpr int kGeoState = 64 + 3;
 constexpr int kGeoRounds = 3;  // parallel rounds of p0 panels before the serial continuation
 
 // Pass 1 of round r: every wavefront evaluates one panel of one unconverged item (round r covers
-// that item's panels done .. done + p0 - 1) and stores each strike's panel sum; in round 0 the
+// that item's panels done .. done + p0 - 1) and stores each strike's panel sum; in the first round the
 // item's last wavefront evaluates E[e^L] instead.
 __global__ __launch_bounds__(256) void k_geo_panels(GeoArgs a, uint32_t round) {
     __shared__ double sh[kGeoWaves][3][64];
