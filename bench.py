@@ -416,6 +416,12 @@ def main():
 
     b32 = ctx.batch(reqs32)
     counts = b32.branch_counts()
+    # every simulated step takes one of the two branches and draws 2 (quadratic) or 1 (exponential)
+    # normals: counts that break this are not counts, and the flop figure built on them is void
+    steps_run = b32.stats()["path_steps"]
+    if (int(counts[0] + counts[1]) != steps_run
+            or int(counts[2] + counts[3]) != 2 * int(counts[0]) + int(counts[1])):
+        raise SystemExit(f"branch counts {counts.tolist()} disagree with the {steps_run} simulated path-steps")
     flops = algorithmic_flops(reqs32, counts)
     log("branch counts (quad, exp, central, tail):", counts.tolist(), "flops/pass %.4e" % flops)
     wall32, kms32 = time_batch(ctx, b32, args.steps, args.warmup)
