@@ -184,7 +184,11 @@ def asian_weights(req: Request, chain: int):
     """(t, w): grid times and weights of expiry `chain`'s Asian average A = sum_j w_j X(t_j), as
     the path kernels accumulate it (host only)."""
     lib = _lib.load()
-    cap = 4 * int(req.steps) * len(req.expiries) + 64
+    # sized from the run schedule: expiry `chain` is finalised after fin_step[chain] grid steps
+    _, fin_step, _ = time_grid(req.expiries, req.n_strikes(), req.steps)
+    if not 0 <= chain < len(fin_step) or fin_step[chain] == 0:
+        raise HexoError(-1, f"asian weights: expiry {chain} is not finalised")
+    cap = int(fin_step[chain]) + 1
     t = np.zeros(cap, np.float64)
     w = np.zeros(cap, np.float64)
     n = lib.hexo_asian_weights(C.byref(req.to_c()), chain, _dp(t), _dp(w), cap)

@@ -82,6 +82,18 @@ static void walk_functionals(const std::vector<HostSeg>& segs, uint32_t flags, F
     }
 }
 
+// Grid steps simulated up to the finalisation of expiry `chain` (-1 when it is never finalised),
+// from the schedule alone: lets callers bound a walk before it allocates per-step arrays.
+static int64_t points_at_fin(const std::vector<HostSeg>& segs, uint32_t chain) {
+    uint64_t p = 0;
+    for (const HostSeg& S : segs) {
+        p += S.n_steps;
+        for (const HostFin& F : S.fins)
+            if (F.chain == chain) return (int64_t)p;
+    }
+    return -1;
+}
+
 int64_t functional_weights(const std::vector<HostSeg>& segs, uint32_t flags, uint32_t chain, std::vector<double>& t,
                            std::vector<double>& w) {
     int64_t n = -1;
@@ -106,6 +118,16 @@ bool plan_geo(const hexo_request* r, const std::vector<HostSeg>& segs, GeoPlan& 
     }
     std::vector<uint32_t> opt_begin(r->n_expiries + 1, 0);
     for (uint32_t c = 0; c < r->n_expiries; ++c) opt_begin[c + 1] = opt_begin[c] + r->n_strikes[c];
+    // the walk below holds every grid step up to the last expiry it reaches: refuse oversized
+    // expiries from the schedule first
+    for (uint32_t c = 0; c < r->n_expiries; ++c) {
+        const int64_t n = r->n_strikes[c] ? points_at_fin(segs, c) : 0;
+        if (n > (int64_t)kGeoMaxPoints) {
+            why = "geometric control variate: expiry " + std::to_string(c) + " has " + std::to_string(n) +
+                  " grid steps (at most " + std::to_string(kGeoMaxPoints) + " per expiry)";
+            return false;
+        }
+    }
     const double rd = (r->flags & HEXO_DRIFT) ? r->r : 0.0;
     bool ok = true;
     walk_functionals(segs, r->flags, [&](const HostFin& F, uint64_t p, const std::vector<double>& t,
@@ -186,6 +208,9 @@ extern "C" int64_t hexo_asian_weights(const hexo_request* r, uint32_t chain, dou
     std::vector<HostSeg> segs;
     uint64_t total = 0;
     if (!build_schedule(r->expiries, r->n_expiries, r->n_strikes, r->steps, segs, total)) return HEXO_ERR_INVALID;
+    // refuse before the walk allocates per-step arrays the caller has no room for
+    const int64_t p = points_at_fin(segs, chain);
+    if (p < 0 || (uint64_t)p + 1 > cap) return HEXO_ERR_INVALID;
     std::vector<double> tt, ww;
     const int64_t n = functional_weights(segs, r->flags, chain, tt, ww);
     if (n < 0) return HEXO_ERR_INVALID;  // never finalised (a trailing expiry without strikes)

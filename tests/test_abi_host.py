@@ -162,3 +162,33 @@ def test_asian_weights_match_oracle(T, strikes, N, flags):
         to, wo = O.asian_weights(T, [len(k) for k in strikes], N, c, flags)
         np.testing.assert_array_equal(t, to)
         np.testing.assert_allclose(w, wo, rtol=0, atol=1e-15)
+
+
+def test_asian_weights_refuse_before_allocating():
+    """A schedule of 2^22 steps per expiry over 4096 expiries (about 2^34 grid steps) is refused from
+    the schedule alone: the weight walks never build per-step arrays the caller has no room for, and
+    control-variate validation refuses the oversized expiry."""
+    import time
+    T = list(np.linspace(0.5, 4096.0, 4096))
+    req = H.Request(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0, expiries=T,
+                    strikes=[[100.0]] * len(T), steps=1 << 22, n_paths=64)
+    lib = _lib.load()
+    t = np.zeros(8)
+    w = np.zeros(8)
+    t0 = time.perf_counter()
+    q = req.to_c()
+    for chain in (0, 4095):
+        n = lib.hexo_asian_weights(C.byref(q), chain, t.ctypes.data_as(C.POINTER(C.c_double)),
+                                   w.ctypes.data_as(C.POINTER(C.c_double)), 8)
+        assert n == -1
+    req.payoffs |= H.ASIAN_GEO_CV
+    assert H.validate(req) == -1
+    assert time.perf_counter() - t0 < 30.0
+    # the Python wrapper sizes its buffers from the schedule (one expiry of 2^22 steps: 32 MiB each)
+    small = H.Request(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0, expiries=[1.0],
+                      strikes=[[100.0]], steps=1 << 22, n_paths=64)
+    tt, ww = H.asian_weights(small, 0)
+    assert len(tt) == len(ww) == (1 << 22) + 1
+    # the grid reaches T = 1 exactly (2^22 steps of 2^-22): the reference's final term replaces the
+    # last trapezoid, so the weights sum to 1 - 1/N
+    assert abs(ww.sum() - (1.0 - 2.0 ** -22)) < 1e-9
