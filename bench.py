@@ -14,6 +14,7 @@ batched uneven-request workload, serving latency, and the CPU baseline (the orac
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -261,22 +262,32 @@ def control_variate_leg(ctx, args, plain_value: float, plain_kernel_ms: float):
     pall, sall = b.results()
     res = list(zip(b.per_request(pall), b.per_request(sall)))
     b.close()
-    factors, atm, opt_factors = [], [], []
+    # the regression-coefficient estimator formed on the GPU (flag CV_REGRESSION), same paths
+    reg_reqs = [dataclasses.replace(q, flags=q.flags | H.CV_REGRESSION) for q in reqs]
+    b = ctx.batch(reg_reqs)
+    b.run_eager()
+    b.sync()
+    preg, sreg = b.results()
+    res_reg = list(zip(b.per_request(preg), b.per_request(sreg)))
+    b.close()
+    factors, atm, opt_factors, reg_dev_vs_host = [], [], [], 0.0
     t0 = time.perf_counter()
     exact = [ctx.geo_asian_price(q) for q in reqs]  # exact geometric prices (each its own batch)
     geo_ms = 1e3 * (time.perf_counter() - t0) / len(reqs)
-    for q, (p, s), e in zip(reqs, res, exact):
+    for q, (p, s), e, (p_reg, s_reg) in zip(reqs, res, exact, res_reg):
         d = q.split(s)
         pr = q.split(p)
         ok = (pr["asian"] > 1e-2) & (d["asian_geo_cv"] > 0)
         factors.extend(list((d["asian"][ok] / d["asian_geo_cv"][ok]) ** 2))
         k = int(np.argmin(np.abs(q.flat_strikes() - q.S0)))
         atm.append(float((d["asian"][k] / d["asian_geo_cv"][k]) ** 2))
-        # the regression-coefficient estimator from the geometric block
-        _, se_opt, _ = H.cv_optimal((pr["asian"], d["asian"]), (pr["geo_asian"], d["geo_asian"]),
-                                    (pr["asian_geo_cv"], d["asian_geo_cv"]), e)
-        okk = ok & (se_opt > 0)
-        opt_factors.extend(list((d["asian"][okk] / se_opt[okk]) ** 2))
+        # the regression-coefficient estimator: the GPU's (flag) against the host's from the blocks
+        p_opt, se_opt, _ = H.cv_optimal((pr["asian"], d["asian"]), (pr["geo_asian"], d["geo_asian"]),
+                                        (pr["asian_geo_cv"], d["asian_geo_cv"]), e)
+        p_dev, se_dev = q.split(p_reg)["asian_geo_cv"], q.split(s_reg)["asian_geo_cv"]
+        reg_dev_vs_host = max(reg_dev_vs_host, float(np.max(np.abs(p_dev - p_opt)) / q.S0))
+        okk = ok & (se_dev > 0)
+        opt_factors.extend(list((d["asian"][okk] / se_dev[okk]) ** 2))
     value = nominal * steps / wall
     return {"workload": "C2 grid, Asian + European + control-variate Asian + geometric from the same FP32 paths",
             "value": value, "kernel_ms": kms, "over_plain": value / plain_value,
@@ -285,7 +296,8 @@ def control_variate_leg(ctx, args, plain_value: float, plain_kernel_ms: float):
             "variance_reduction_min": float(np.min(factors)),
             "variance_reduction_atm_by_maturity": atm,
             "variance_reduction_regression_coefficient": {"median": float(np.median(opt_factors)),
-                                                           "min": float(np.min(opt_factors))},
+                                                           "min": float(np.min(opt_factors)),
+                                                           "gpu_vs_host_max_abs_over_S0": reg_dev_vs_host},
             "exact_price_ms_per_request": geo_ms,
             "effective_path_steps_per_s_median": value * float(np.median(factors))}
 

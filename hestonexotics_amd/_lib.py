@@ -17,7 +17,7 @@ ERRORS = {
     -9: "HEXO_ERR_STATE", -10: "HEXO_ERR_INJECTED",
 }
 ASIAN, EUROPEAN, ASIAN_GEO_CV, GEO_ASIAN = 1, 2, 4, 8
-DRIFT, DISCOUNT, TRAPEZOID, MARTINGALE = 1, 2, 4, 8
+DRIFT, DISCOUNT, TRAPEZOID, MARTINGALE, CV_REGRESSION = 1, 2, 4, 8, 16
 FP64, FP32, FP32_COMPENSATED = 0, 1, 2
 
 

@@ -16,13 +16,13 @@ from typing import Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import (ASIAN, ASIAN_GEO_CV, CALIB_INITIAL_GIVEN, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED,
+from ._lib import (ASIAN, ASIAN_GEO_CV, CALIB_INITIAL_GIVEN, CV_REGRESSION, DISCOUNT, DRIFT, EUROPEAN, FP32, FP32_COMPENSATED,
                    FP64, GEO_ASIAN, MARTINGALE, SWIFT_FIXED_GRID, TRAPEZOID, HexoError, hexo_batch_stats, hexo_calib_chain,
                    hexo_calib_config, hexo_calib_result, hexo_request, hexo_server_config, hexo_server_stats,
                    hexo_swift_grid, hexo_swift_request, hexo_ticket_info, hexo_tuning)
 
 __all__ = ["Request", "Context", "Batch", "ASIAN", "EUROPEAN", "ASIAN_GEO_CV", "GEO_ASIAN", "DRIFT", "DISCOUNT",
-           "TRAPEZOID", "MARTINGALE", "FP64", "FP32", "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "asian_weights",
+           "TRAPEZOID", "MARTINGALE", "CV_REGRESSION", "FP64", "FP32", "FP32_COMPENSATED", "HexoError", "validate", "time_grid", "asian_weights",
            "SwiftChain", "SwiftCall", "swift_grid", "cv_optimal"]
 
 GRID_FIELDS = ("m", "exp2_m", "sqrt_exp2_m", "lower", "upper", "k1", "k2", "J")

@@ -112,7 +112,10 @@ std::string validate(const hexo_request* r) {
         if (!fin(r->strikes[i])) return "strikes must be finite";
     if (r->payoffs == 0 || (r->payoffs & ~15u))
         return "payoffs must be a non-empty subset of ASIAN|EUROPEAN|ASIAN_GEO_CV|GEO_ASIAN";
-    if (r->flags & ~15u) return "unknown flag bits";
+    if (r->flags & ~31u) return "unknown flag bits";
+    if ((r->flags & HEXO_CV_REGRESSION) &&
+        (r->payoffs & (HEXO_ASIAN | HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN)) != (HEXO_ASIAN | HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN))
+        return "HEXO_CV_REGRESSION needs the ASIAN, ASIAN_GEO_CV and GEO_ASIAN payoffs";
     if (r->precision > 2u) return "precision must be HEXO_FP64, HEXO_FP32 or HEXO_FP32_COMPENSATED";
     const uint64_t n_tiles = (r->n_paths + kTile - 1) / kTile;
     if (n_tiles * n_opts * n_payoffs(r->payoffs) > (1ull << 33)) return "n_paths * outputs too large for one request";

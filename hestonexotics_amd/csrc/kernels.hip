@@ -31,6 +31,7 @@ constexpr uint32_t kAsianGeoCV = 4u;
 constexpr uint32_t kGeoAsian = 8u;
 constexpr uint32_t kTrapezoid = 4u;
 constexpr uint32_t kMartingale = 8u;
+constexpr uint32_t kCvRegression = 16u;
 constexpr uint32_t kPrecFP64 = 0u;
 constexpr uint32_t kPrecComp = 2u;
 
@@ -786,18 +787,13 @@ __global__ __launch_bounds__(256) void k_reduce_groups(ReduceArgs a, uint64_t to
     }
 }
 
-// One wavefront per output: lane l sums groups l, l + 64, l + 128, ... in order and the 64 lane
-// sums are combined by a fixed xor butterfly (every lane ends with the same bits). The order
-// depends only on the request's group count, never on the launch.
-__global__ __launch_bounds__(256) void k_reduce_final(ReduceArgs a, uint64_t total) {
-    const uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / (uint64_t)kTile;  // wave-uniform
-    const uint32_t lane = threadIdx.x % (uint32_t)kTile;
-    if (t >= total) return;
-    const uint32_t ri = find_req(a.out_end, a.n_req, t);
-    const uint32_t o = (uint32_t)(t - (ri ? a.out_end[ri - 1] : 0ull));
-    const DevReq& R = a.reqs[ri];
+// Sum and sum of squares of output o over the request's groups, on one wavefront: lane l sums
+// groups l, l + 64, l + 128, ... in order and the 64 lane sums are combined by a fixed xor
+// butterfly (every lane ends with the same bits). The order depends only on the request's group
+// count, never on the launch.
+__device__ __forceinline__ void output_sums(const ReduceArgs& a, const DevReq& R, uint32_t o, uint32_t lane,
+                                            double& s, double& ss) {
     const uint32_t n_out = R.n_opts * R.n_pay;
-    double s, ss;
     if (R.precision == kPrecComp) {
         const float4* p = reinterpret_cast<const float4*>(a.groups);
         CompF cs{0.f, 0.f}, css{0.f, 0.f};
@@ -830,17 +826,53 @@ __global__ __launch_bounds__(256) void k_reduce_final(ReduceArgs a, uint64_t tot
             ss += __shfl_xor(ss, m);
         }
     }
-    if (lane != 0) return;
+}
+
+// Sample mean and variance (n - 1 denominator, clamped at 0) from a sum and a sum of squares.
+__device__ __forceinline__ void mean_var(double s, double ss, double n, double& mean, double& var) {
+    mean = s / n;
+    var = n > 1.0 ? (ss - n * mean * mean) / (n - 1.0) : 0.0;
+    var = var > 0.0 ? var : 0.0;
+}
+
+// One wavefront per output. A control-variate output adds the exact geometric price to the mean of
+// A - G; with kCvRegression it is A - beta (G - E[G]) instead, beta = Cov(A, G) / Var G from the
+// same paths' Asian (block 0) and geometric outputs of the option: Cov from
+// Var(A - G) = Var A + Var G - 2 Cov(A, G), variance Var A - beta Cov(A, G).
+__global__ __launch_bounds__(256) void k_reduce_final(ReduceArgs a, uint64_t total) {
+    const uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / (uint64_t)kTile;  // wave-uniform
+    const uint32_t lane = threadIdx.x % (uint32_t)kTile;
+    if (t >= total) return;
+    const uint32_t ri = find_req(a.out_end, a.n_req, t);
+    const uint32_t o = (uint32_t)(t - (ri ? a.out_end[ri - 1] : 0ull));
+    const DevReq& R = a.reqs[ri];
     const uint32_t opt = o % R.n_opts;
+    const bool cv = a.geo && payoff_kind(R.payoffs, o / R.n_opts) == kAsianGeoCV;
+    const bool reg = cv && (R.flags & kCvRegression);  // request-uniform
+    double s, ss, sa = 0.0, ssa = 0.0, sg = 0.0, ssg = 0.0;
+    output_sums(a, R, o, lane, s, ss);
+    if (reg) {
+        output_sums(a, R, opt, lane, sa, ssa);
+        output_sums(a, R, (uint32_t)__popc(R.payoffs & (kGeoAsian - 1u)) * R.n_opts + opt, lane, sg, ssg);
+    }
+    if (lane != 0) return;
     const double disc = a.opts[R.opt_begin + opt].disc;
     const double scale = (R.precision == kPrecFP64) ? 1.0 : R.S0;
     const double n = (double)R.n_paths;
-    const double mean = s / n;
-    double var = n > 1.0 ? (ss - n * mean * mean) / (n - 1.0) : 0.0;
-    var = var > 0.0 ? var : 0.0;
-    // control variate: mean of (arithmetic - geometric payoff) plus the exact geometric price
-    const bool cv = a.geo && payoff_kind(R.payoffs, o / R.n_opts) == kAsianGeoCV;
-    a.prices[R.out_base + o] = disc * (scale * mean + (cv ? a.geo[R.out_base + o] : 0.0));
+    double mean, var;
+    mean_var(s, ss, n, mean, var);
+    double price = scale * mean + (cv ? a.geo[R.out_base + o] : 0.0);
+    if (reg) {
+        double ma, va, mg, vg;
+        mean_var(sa, ssa, n, ma, va);
+        mean_var(sg, ssg, n, mg, vg);
+        const double cov = 0.5 * (va + vg - var);
+        const double beta = vg > 0.0 ? cov / vg : 0.0;
+        price = scale * ma - beta * (scale * mg - a.geo[R.out_base + o]);
+        var = va - beta * cov;
+        var = var > 0.0 ? var : 0.0;
+    }
+    a.prices[R.out_base + o] = disc * price;
     if (a.std_errs) a.std_errs[R.out_base + o] = scale * disc * ::sqrt(var / n);
 }
 

@@ -50,8 +50,8 @@ typedef enum hexo_status {
  * expiry's weight sum W >= 0.25 (at least 2 steps). Outputs are payoff-major in bit order. */
 #define HEXO_ASIAN_GEO_CV 4u
 /* The geometric-average call of the same functional, max(G - K, 0), by plain Monte Carlo. With
- * HEXO_ASIAN, HEXO_ASIAN_GEO_CV and hexo_geo_asian_price it gives the variances and covariance for
- * the control variate with the regression coefficient instead of 1 (never worse than plain). */
+ * HEXO_ASIAN and HEXO_ASIAN_GEO_CV it gives the variances and covariance for the control variate
+ * with the regression coefficient instead of 1 (flag HEXO_CV_REGRESSION). */
 #define HEXO_GEO_ASIAN 8u
 
 /* flags: documented extensions; 0 reproduces the reference exactly */
@@ -64,6 +64,12 @@ typedef enum hexo_status {
  * the expectation does not exist (A >= 1/(2a) in the quadratic branch, A >= beta in the
  * exponential one) that step keeps the reference's K0. */
 #define HEXO_MARTINGALE 8u
+/* The control variate with the regression coefficient instead of 1: the HEXO_ASIAN_GEO_CV block
+ * holds A - beta (G - E[G]) with beta = Cov(A, G) / Var G estimated from the same paths, and its
+ * standard error sqrt((Var A - beta Cov(A, G)) / n), never above the plain Asian one. Needs
+ * HEXO_ASIAN, HEXO_ASIAN_GEO_CV and HEXO_GEO_ASIAN in payoffs (the covariance comes from the
+ * variances of A, G and A - G); formed in the final reduction on the GPU. */
+#define HEXO_CV_REGRESSION 16u
 
 /* precision of the path kernel */
 #define HEXO_FP64 0u             /* plain FP64 kernel: reference form, FP64 everywhere */
@@ -93,7 +99,7 @@ typedef struct hexo_request {
     const uint32_t* n_strikes;
     const double* strikes;
     uint32_t payoffs;   /* HEXO_ASIAN | HEXO_EUROPEAN | HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN */
-    uint32_t flags;     /* HEXO_DRIFT | HEXO_DISCOUNT | HEXO_TRAPEZOID | HEXO_MARTINGALE */
+    uint32_t flags;     /* HEXO_DRIFT | HEXO_DISCOUNT | HEXO_TRAPEZOID | HEXO_MARTINGALE | HEXO_CV_REGRESSION */
     uint32_t precision; /* HEXO_FP64 | HEXO_FP32 | HEXO_FP32_COMPENSATED */
     uint32_t reserved;  /* must be 0 */
 } hexo_request;

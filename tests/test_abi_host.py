@@ -54,11 +54,16 @@ MUTATIONS = {
     "no_strikes": lambda r: setattr(r, "strikes", [[]]),
     "payoffs": lambda r: setattr(r, "payoffs", 0),
     "payoff_bits": lambda r: setattr(r, "payoffs", 16),
-    "flags": lambda r: setattr(r, "flags", 64),
+    "flags": lambda r: setattr(r, "flags", 32),
     "precision": lambda r: setattr(r, "precision", 7),
     "path_offset_unaligned": lambda r: setattr(r, "path_offset", 100),
     # the control variate needs an Asian average with weight sum >= 1/4 (one step has 0)
     "control_variate_one_step": lambda r: (setattr(r, "payoffs", H.ASIAN_GEO_CV), setattr(r, "steps", 1)),
+    # the regression coefficient needs the Asian and geometric blocks beside the control variate
+    "cv_regression_without_geo": lambda r: (setattr(r, "payoffs", H.ASIAN | H.ASIAN_GEO_CV),
+                                            setattr(r, "flags", H.CV_REGRESSION)),
+    "cv_regression_without_cv": lambda r: (setattr(r, "payoffs", H.ASIAN | H.GEO_ASIAN),
+                                           setattr(r, "flags", H.CV_REGRESSION)),
 }
 
 
@@ -66,6 +71,9 @@ def test_validation_accepts_good_request():
     assert H.validate(good()) == 0
     r = good()
     r.expiries, r.strikes = [0.3, 0.31, 1.0], [[90, 100], [], [100]]
+    assert H.validate(r) == 0
+    r.payoffs = H.ASIAN | H.EUROPEAN | H.ASIAN_GEO_CV | H.GEO_ASIAN
+    r.flags = H.CV_REGRESSION | H.MARTINGALE | H.DISCOUNT | H.DRIFT | H.TRAPEZOID
     assert H.validate(r) == 0
 
 

@@ -10,6 +10,8 @@
 (3) FP32 against FP64 on identical draws within 1e-4; the standard error is a small fraction of the
     plain Asian estimator's, and the two estimators agree within the plain standard error.
 (4) Mixed batches of all kernel classes return what each request returns alone.
+(5) The regression-coefficient estimator formed on the GPU (flag CV_REGRESSION) equals the host's
+    cv_optimal over the same paths' blocks.
 """
 import dataclasses
 
@@ -162,6 +164,28 @@ def test_geometric_block_and_regression_estimator(ctx, oracle, name):
     assert np.all(np.abs(price - p[:n]) < 4 * s[:n] + 1e-12)
 
 
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32, H.FP32_COMPENSATED])
+@pytest.mark.parametrize("name", sorted(geo_requests()))
+def test_regression_flag_equals_host_estimator(ctx, name, precision):
+    """CV_REGRESSION forms A - beta (G - E[G]) in the final reduction on the GPU: its block equals
+    cv_optimal applied to the same paths' Asian, control-variate and geometric blocks priced
+    without the flag; the other blocks are unchanged bit for bit."""
+    say("gpu cv regression flag", name, precision)
+    req = cv(geo_requests()[name], precision, H.ASIAN | H.EUROPEAN | H.ASIAN_GEO_CV | H.GEO_ASIAN)
+    p, s = ctx.price(req)
+    pr, sr = ctx.price(cv(req, precision, req.payoffs, H.CV_REGRESSION))
+    n = req.n_opts
+    exact = ctx.geo_asian_price(req)
+    price, se, _ = H.cv_optimal((p[:n], s[:n]), (p[3 * n:], s[3 * n:]), (p[2 * n:3 * n], s[2 * n:3 * n]), exact)
+    for blk in (0, 1, 3):
+        np.testing.assert_array_equal(pr[blk * n:(blk + 1) * n], p[blk * n:(blk + 1) * n])
+        np.testing.assert_array_equal(sr[blk * n:(blk + 1) * n], s[blk * n:(blk + 1) * n])
+    np.testing.assert_allclose(pr[2 * n:3 * n], price, rtol=1e-12, atol=1e-12 * req.S0)
+    # the standard error is a difference of variances: compared on the variance scale
+    np.testing.assert_allclose(sr[2 * n:3 * n] ** 2, se ** 2, rtol=1e-9, atol=1e-12 * s[:n].max() ** 2)
+    assert np.all(sr[2 * n:3 * n] <= s[:n] * (1 + 1e-9))
+
+
 def test_exact_prices_over_the_uneven_request_distribution(ctx, oracle):
     """The 512 uneven requests' distribution (kappa up to 20, sigma up to 1, T from 0.1 to 5 on
     252 steps a year) with the control variate, in one batch: every exact geometric price is
@@ -194,6 +218,7 @@ def test_mixed_batch_equals_single(ctx):
         cv(geo_requests()["stiff_c5"], H.FP32, H.ASIAN_GEO_CV, H.MARTINGALE),
         cv(geo_requests()["peaked"], H.FP32, H.GEO_ASIAN),
         cv(geo_requests()["positive_rho"], H.FP64, H.EUROPEAN | H.GEO_ASIAN),
+        cv(geo_requests()["wide_strikes"], H.FP32, H.ASIAN | H.ASIAN_GEO_CV | H.GEO_ASIAN, H.CV_REGRESSION),
     ]
     together = ctx.price_batch(variants)
     for r, (p, s) in zip(variants, together):
