@@ -302,6 +302,66 @@ def control_variate_leg(ctx, args, plain_value: float, plain_kernel_ms: float):
             "effective_path_steps_per_s_median": value * float(np.median(factors))}
 
 
+def price_errors(ctx, reqs, p32, s32, p64, report_path=None):
+    """The error of every grid price: FP32 against the FP64 kernel and against the oracle's CPU FP64
+    Monte Carlo, both on the same draws (relative for prices above 0.01, and in standard errors);
+    European prices also against SWIFT on the same model (FP64; the difference includes the QE
+    scheme's time-discretisation bias at 52 steps a year). Returns a per-maturity summary;
+    report_path: one CSV row per price."""
+    from oracle import oracle as O
+    import hestonexotics_amd as H
+
+    def in_se(diff, se):  # |diff| / se; a price whose paths all pay 0 has se = 0 and diff = 0
+        diff = np.abs(np.asarray(diff, np.float64))
+        return np.where(se > 0, diff / np.where(se > 0, se, 1.0), np.where(diff == 0, 0.0, np.inf))
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    chains = [H.SwiftChain(r.v0, r.theta, r.rho, r.kappa, r.sigma, S0=r.S0, T=r.expiries[0], strikes=r.strikes[0],
+                           r=r.r) for r in reqs]
+    swift = [q for q, _, _ in ctx.swift_price(chains)]
+    rows, summary, o = [], [], 0
+    for r, sw in zip(reqs, swift):
+        n = r.n_outputs
+        g32, e32, g64 = p32[o:o + n], s32[o:o + n], p64[o:o + n]
+        o += n
+        cpu = O.price(r.params, r.S0, r.expiries, r.n_strikes(), r.flat_strikes(), steps=r.steps,
+                      n_paths=r.n_paths, seed=r.seed, r=r.r, flags=r.flags, payoffs=r.payoffs,
+                      n_threads=threads, kind="fast")["prices"]
+        k = r.n_opts
+        entry = {"T": r.expiries[0], "steps": r.steps}
+        for blk, name in enumerate(("asian", "european")):
+            sl = slice(blk * k, (blk + 1) * k)
+            big = g64[sl] > 1e-2
+            d = {"max_rel_vs_fp64_gpu": float(np.max(np.abs(g32[sl] - g64[sl])[big] / g64[sl][big], initial=0.0)),
+                 "max_rel_vs_cpu_fp64": float(np.max(np.abs(g32[sl] - cpu[sl])[big] / cpu[sl][big], initial=0.0)),
+                 "max_se_vs_cpu_fp64": float(np.max(in_se(g32[sl] - cpu[sl], e32[sl])))}
+            if name == "european":
+                # in standard errors for prices above 0.01 (far out of the money few or no paths pay
+                # and the standard error says nothing); the absolute difference over all strikes
+                d["max_se_vs_swift"] = float(np.max(in_se(g32[sl] - sw, e32[sl])[big], initial=0.0))
+                d["max_abs_vs_swift"] = float(np.max(np.abs(g32[sl] - sw)))
+            entry[name] = d
+            for j, K in enumerate(r.strikes[0]):
+                i = blk * k + j
+                rows.append((r.expiries[0], K, name, g32[i], e32[i], g64[i], cpu[i],
+                             sw[j] if name == "european" else float("nan")))
+        summary.append(entry)
+    if report_path:
+        with open(report_path, "w") as f:
+            f.write("T,K,payoff,fp32_price,fp32_se,fp64_gpu_price,fp64_cpu_price,swift_price,"
+                    "rel_vs_fp64_gpu,se_units_vs_cpu,se_units_vs_swift\n")
+            for T, K, name, a, e, b, c, w in rows:
+                f.write(f"{T},{K:.6f},{name},{a:.12g},{e:.6g},{b:.12g},{c:.12g},{w:.12g},"
+                        f"{abs(a - b) / max(b, 1e-300):.3e},{float(in_se(a - c, e)):.3e},"
+                        f"{float(in_se(a - w, e)):.3e}\n")
+    se_cpu = max(max(m[n]["max_se_vs_cpu_fp64"] for n in ("asian", "european")) for m in summary)
+    se_sw = max(m["european"]["max_se_vs_swift"] for m in summary)
+    return {"n_prices": len(rows), "max_se_vs_cpu_fp64": se_cpu, "all_within_3se_of_cpu_fp64": bool(se_cpu <= 3.0),
+            "max_se_european_vs_swift_above_0.01": se_sw, "european_within_3se_of_swift_above_0.01": bool(se_sw <= 3.0),
+            "max_abs_european_vs_swift": max(m["european"]["max_abs_vs_swift"] for m in summary),
+            "per_maturity": summary}
+
+
 def time_batch(ctx, batch, steps: int, warmup: int):
     for _ in range(warmup):
         batch.run_eager()
@@ -333,6 +393,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--paths", type=int, default=1_000_000, help="paths per maturity")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-price-errors", action="store_true",
+                    help="skip the per-price error check (oracle CPU FP64 Monte Carlo and SWIFT on the grid)")
+    ap.add_argument("--price-report", default=None,
+                    help="CSV file for the per-price errors of the grid workload (one row per price)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-fp64-scale", type=float, default=0.05)
@@ -570,6 +634,9 @@ def main():
                           "batches": st["batches"], "max_queue_depth": st["max_queue_depth"]}
         log("serving:", out["serving"])
 
+    if not args.no_price_errors:
+        out["price_errors"] = price_errors(ctx, reqs32, p32, s32, p64, args.price_report)
+        log("price errors:", {k: v for k, v in out["price_errors"].items() if k != "per_maturity"})
     if not args.no_cpu:
         cb = cpu_baseline(W.c2(n_paths=args.paths, precision=H.FP64), args.cpu_seconds)
         out["cpu_baseline"] = cb
