@@ -122,7 +122,7 @@ int hexo_ctx_create(int device, hexo_ctx** out) {
 
 int hexo_ctx_destroy(hexo_ctx* ctx) {
     if (!ctx) return HEXO_ERR_INVALID;
-    if (ctx->server) (void)hexo_server_stop(ctx);
+    if (ctx->server_ref()) (void)hexo_server_stop(ctx);
     (void)hipSetDevice(ctx->device);
     {
         // batches still alive lose their device state here; their handles stay valid for
@@ -141,7 +141,14 @@ int hexo_ctx_destroy(hexo_ctx* ctx) {
     return HEXO_OK;
 }
 
-const char* hexo_ctx_last_error(hexo_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+const char* hexo_ctx_last_error(hexo_ctx* ctx) {
+    if (!ctx) return "no context";
+    // a per-thread copy: another thread's failure may replace the context's message meanwhile
+    thread_local std::string copy;
+    std::lock_guard<std::mutex> g(ctx->err_mu);
+    copy = ctx->last_error;
+    return copy.c_str();
+}
 
 int hexo_device_info(hexo_ctx* ctx, char* name, size_t name_len, int* n_cu, int* clock_khz) {
     if (!ctx) return no_ctx();

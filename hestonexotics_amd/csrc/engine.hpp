@@ -149,8 +149,12 @@ struct hexo_ctx {
     hipStream_t stream = nullptr;
     hexo_tuning tuning{2, 256, 0, 0};
     std::recursive_mutex mu;
+    std::mutex err_mu;  // guards last_error: failures may be reported from several threads at once
     std::string last_error;
-    struct hexo_server_state* server = nullptr;  // owned; see server.cpp
+    // the serving loop (server.cpp); every server call holds a reference while it runs, so a
+    // concurrent hexo_server_stop never frees the state under it. Guarded by server_mu.
+    std::mutex server_mu;
+    std::shared_ptr<struct hexo_server_state> server;
     std::unordered_set<struct hexo_batch*> batches;  // live batches; released by hexo_ctx_destroy
     // SWIFT pricer workspace, grown on demand and kept for the context's lifetime (no per-call
     // allocation, which would also synchronise the device)
@@ -158,8 +162,13 @@ struct hexo_ctx {
     size_t swift_dev_bytes = 0;
     hexo::PinnedBuf swift_host;
     int set_error(int code, const std::string& msg) {
+        std::lock_guard<std::mutex> g(err_mu);
         last_error = msg;
         return code;
+    }
+    std::shared_ptr<struct hexo_server_state> server_ref() {
+        std::lock_guard<std::mutex> g(server_mu);
+        return server;
     }
 };
 

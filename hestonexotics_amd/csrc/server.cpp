@@ -211,10 +211,10 @@ extern "C" {
 int hexo_server_start(hexo_ctx* ctx, const hexo_server_config* cfg) {
     if (!ctx) return HEXO_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    if (ctx->server) return ctx->set_error(HEXO_ERR_STATE, "server already running");
+    if (ctx->server_ref()) return ctx->set_error(HEXO_ERR_STATE, "server already running");
     if (cfg && (cfg->streams > 4)) return ctx->set_error(HEXO_ERR_INVALID, "streams must be <= 4");
     if (hipSetDevice(ctx->device) != hipSuccess) return ctx->set_error(HEXO_ERR_HIP, "hipSetDevice failed");
-    auto* s = new hexo_server_state();
+    auto s = std::make_shared<hexo_server_state>();
     s->ctx = ctx;
     if (cfg) s->cfg = *cfg;
     if (s->cfg.streams == 0) s->cfg.streams = 2;
@@ -224,10 +224,7 @@ int hexo_server_start(hexo_ctx* ctx, const hexo_server_config* cfg) {
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
         budget = std::min<size_t>(fr / 4, (size_t)8 << 30);
     }
-    if (!s->arena.init(budget)) {
-        delete s;
-        return ctx->set_error(HEXO_ERR_NOMEM, "could not reserve the serving memory budget");
-    }
+    if (!s->arena.init(budget)) return ctx->set_error(HEXO_ERR_NOMEM, "could not reserve the serving memory budget");
     s->budget = s->arena.capacity();
     s->st.budget_bytes = s->budget;
     s->streams.resize(s->cfg.streams);
@@ -238,37 +235,42 @@ int hexo_server_start(hexo_ctx* ctx, const hexo_server_config* cfg) {
             st = nullptr;
             for (auto& t : s->streams)
                 if (t) (void)hipStreamDestroy(t);
-            delete s;
             return ctx->set_error(HEXO_ERR_HIP, "hipStreamCreate failed");
         }
-    ctx->server = s;
-    s->worker = std::thread([s] { s->run(); });
+    // the worker never outlives the state: hexo_server_stop joins it before releasing its reference
+    hexo_server_state* raw = s.get();
+    s->worker = std::thread([raw] { raw->run(); });
+    std::lock_guard<std::mutex> gs(ctx->server_mu);
+    ctx->server = std::move(s);
     return HEXO_OK;
 }
 
 int hexo_server_stop(hexo_ctx* ctx) {
     if (!ctx) return HEXO_ERR_INVALID;
-    hexo_server_state* s = nullptr;
+    std::shared_ptr<hexo_server_state> s;
     {
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
-        s = ctx->server;
-        if (!s) return ctx->set_error(HEXO_ERR_STATE, "server not running");
-        ctx->server = nullptr;
+        std::lock_guard<std::mutex> gs(ctx->server_mu);
+        s = std::move(ctx->server);
+        ctx->server.reset();
     }
+    if (!s) return ctx->set_error(HEXO_ERR_STATE, "server not running");
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stopping = true;
         s->cv.notify_all();
     }
-    s->worker.join();
+    s->worker.join();  // drains the queue and the batches in flight
     for (auto& st : s->streams) (void)hipStreamDestroy(st);
-    delete s;
+    s->streams.clear();
+    s->arena.release();
+    // calls still holding a reference (a wait on a finished ticket) end on the state they hold
     return HEXO_OK;
 }
 
 int hexo_submit(hexo_ctx* ctx, const hexo_request* req, uint64_t* ticket) {
     if (!ctx || !ticket) return HEXO_ERR_INVALID;
-    hexo_server_state* s = ctx->server;
+    const std::shared_ptr<hexo_server_state> s = ctx->server_ref();
     if (!s) return ctx->set_error(HEXO_ERR_STATE, "server not running");
     std::string why = hexo::validate(req);
     std::lock_guard<std::mutex> lk(s->mu);
@@ -307,7 +309,7 @@ int hexo_submit(hexo_ctx* ctx, const hexo_request* req, uint64_t* ticket) {
 
 int hexo_poll(hexo_ctx* ctx, uint64_t ticket, int* done) {
     if (!ctx || !done) return HEXO_ERR_INVALID;
-    hexo_server_state* s = ctx->server;
+    const std::shared_ptr<hexo_server_state> s = ctx->server_ref();
     if (!s) return ctx->set_error(HEXO_ERR_STATE, "server not running");
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->tickets.find(ticket);
@@ -319,7 +321,7 @@ int hexo_poll(hexo_ctx* ctx, uint64_t ticket, int* done) {
 int hexo_wait(hexo_ctx* ctx, uint64_t ticket, double timeout_s, double* prices, double* std_errs,
               hexo_ticket_info* info) {
     if (!ctx) return HEXO_ERR_INVALID;
-    hexo_server_state* s = ctx->server;
+    const std::shared_ptr<hexo_server_state> s = ctx->server_ref();
     if (!s) return ctx->set_error(HEXO_ERR_STATE, "server not running");
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->tickets.find(ticket);
@@ -342,7 +344,7 @@ int hexo_wait(hexo_ctx* ctx, uint64_t ticket, double timeout_s, double* prices, 
 
 int hexo_cancel(hexo_ctx* ctx, uint64_t ticket) {
     if (!ctx) return HEXO_ERR_INVALID;
-    hexo_server_state* s = ctx->server;
+    const std::shared_ptr<hexo_server_state> s = ctx->server_ref();
     if (!s) return ctx->set_error(HEXO_ERR_STATE, "server not running");
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->tickets.find(ticket);
@@ -357,7 +359,7 @@ int hexo_cancel(hexo_ctx* ctx, uint64_t ticket) {
 
 int hexo_server_stats_get(hexo_ctx* ctx, hexo_server_stats* out) {
     if (!ctx || !out) return HEXO_ERR_INVALID;
-    hexo_server_state* s = ctx->server;
+    const std::shared_ptr<hexo_server_state> s = ctx->server_ref();
     if (!s) return ctx->set_error(HEXO_ERR_STATE, "server not running");
     std::lock_guard<std::mutex> lk(s->mu);
     *out = s->st;
@@ -369,7 +371,7 @@ int hexo_server_stats_get(hexo_ctx* ctx, hexo_server_stats* out) {
 
 int hexo_inject_fault(hexo_ctx* ctx, uint32_t kind, uint32_t count) {
     if (!ctx || kind > 1) return HEXO_ERR_INVALID;
-    hexo_server_state* s = ctx->server;
+    const std::shared_ptr<hexo_server_state> s = ctx->server_ref();
     if (!s) return ctx->set_error(HEXO_ERR_STATE, "server not running");
     std::lock_guard<std::mutex> lk(s->mu);
     if (kind == 0)
@@ -382,7 +384,7 @@ int hexo_inject_fault(hexo_ctx* ctx, uint32_t kind, uint32_t count) {
 int hexo_server_run_trace(hexo_ctx* ctx, const hexo_request* reqs, const double* arrival_s, uint32_t n,
                           double* prices, double* std_errs, hexo_ticket_info* info) {
     if (!ctx || (!reqs && n) || (!arrival_s && n)) return HEXO_ERR_INVALID;
-    if (!ctx->server) return ctx->set_error(HEXO_ERR_STATE, "server not running");
+    if (!ctx->server_ref()) return ctx->set_error(HEXO_ERR_STATE, "server not running");
     for (uint32_t i = 0; i < n; ++i) {
         std::string why = hexo::validate(&reqs[i]);
         if (!why.empty()) return ctx->set_error(HEXO_ERR_INVALID, "request " + std::to_string(i) + ": " + why);

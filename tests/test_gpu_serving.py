@@ -126,3 +126,54 @@ def test_injected_launch_failure_is_reported_and_server_recovers(ctx):
         assert codes[0] == -10 and all(c == 0 for c in codes[1:]), codes
     finally:
         ctx.server_stop()
+
+
+def test_stop_while_clients_submit_and_wait(ctx):
+    """hexo_server_stop while four threads submit and wait (ctypes releases the GIL, so the calls
+    overlap): every call returns a status, each collected result equals the request priced alone,
+    and calls after the stop report that the server is not running."""
+    import dataclasses
+    import threading
+    import time
+    say("gpu serving stop under concurrent clients")
+    base = W.c1(n_paths=2000)
+    ref, _ = ctx.price(base)
+    ctx.server_start(streams=2)
+    stop = threading.Event()
+    results = []
+    lock = threading.Lock()
+
+    def client():
+        req = dataclasses.replace(base)  # each thread keeps its own argument arrays alive
+        while not stop.is_set():
+            try:
+                t = ctx.submit(req)
+            except H.HexoError as e:
+                with lock:
+                    results.append(("submit", e.code, None))
+                if e.code == -9:  # HEXO_ERR_STATE: the server has stopped
+                    return
+                continue
+            try:
+                p, _, _ = ctx.wait(t, req.n_outputs)
+                with lock:
+                    results.append(("wait", 0, bool(np.array_equal(p, ref))))
+            except H.HexoError as e:
+                with lock:
+                    results.append(("wait", e.code, None))
+
+    threads = [threading.Thread(target=client) for _ in range(4)]
+    for th in threads:
+        th.start()
+    time.sleep(0.5)
+    ctx.server_stop()
+    stop.set()
+    for th in threads:
+        th.join(timeout=60)
+    assert not any(th.is_alive() for th in threads)
+    done = [r for r in results if r[0] == "wait" and r[1] == 0]
+    say(f"  {len(done)} results collected, {len(results) - len(done)} calls refused")
+    assert done and all(r[2] for r in done)
+    assert all(r[1] in (0, -9) for r in results), sorted(set(r[1] for r in results))
+    with pytest.raises(H.HexoError):
+        ctx.submit(base)
