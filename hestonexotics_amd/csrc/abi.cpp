@@ -401,8 +401,7 @@ int hexo_normals(hexo_ctx* ctx, const uint32_t* words, uint64_t n, uint32_t prec
 
 int hexo_get_tuning(hexo_ctx* ctx, hexo_tuning* t) {
     if (!ctx || !t) return HEXO_ERR_INVALID;
-    std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    *t = ctx->tuning;
+    *t = ctx->tuning_now();
     return HEXO_OK;
 }
 
@@ -412,9 +411,7 @@ int hexo_set_tuning(hexo_ctx* ctx, const hexo_tuning* t) {
     if (!(p == 1 || p == 2 || p == 4)) return ctx->set_error(HEXO_ERR_INVALID, "paths_per_thread must be 1, 2 or 4");
     if (!(b == 64 || b == 128 || b == 256)) return ctx->set_error(HEXO_ERR_INVALID, "block_threads must be 64, 128 or 256");
     if (t->waves_per_simd > 8) return ctx->set_error(HEXO_ERR_INVALID, "waves_per_simd must be <= 8");
-    std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    ctx->tuning = *t;
-    ctx->tuning.reserved = 0;
+    ctx->set_tuning(*t);
     return HEXO_OK;
 }
 
@@ -424,8 +421,9 @@ int hexo_autotune(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, uint32_t 
     if (reps == 0) reps = 1;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     DeviceGuard dg(ctx->device);
-    const hexo_tuning saved = ctx->tuning;
-    hexo_tuning winner = saved;
+    // trial shapes are passed to the batches directly: the context's shape (which the serving
+    // thread may be reading) changes once, to the winner
+    hexo_tuning winner = ctx->tuning_now();
     double best_ms = 1e300;
     const uint32_t ppts[] = {1, 2, 4};
     const uint32_t blocks[] = {64, 128, 256};
@@ -433,14 +431,11 @@ int hexo_autotune(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, uint32_t 
     for (uint32_t p : ppts)
         for (uint32_t b : blocks)
             for (uint32_t w : waves) {
-                ctx->tuning = hexo_tuning{p, b, w, 0};
+                const hexo_tuning trial{p, b, w, 0};
                 std::string err;
                 std::unique_ptr<hexo::Batch> bt;
-                int rc = hexo::batch_create(ctx, reqs, n, ctx->stream, nullptr, bt, err);
-                if (rc != HEXO_OK) {
-                    ctx->tuning = saved;
-                    return ctx->set_error(rc, err);
-                }
+                int rc = hexo::batch_create(ctx, reqs, n, ctx->stream, nullptr, bt, err, nullptr, nullptr, &trial);
+                if (rc != HEXO_OK) return ctx->set_error(rc, err);
                 double total = 0.0;
                 for (uint32_t r = 0; r <= reps && rc == HEXO_OK; ++r) {  // first run warms up
                     rc = hexo::batch_enqueue(*bt, PathMode::Normal, true, nullptr, err);
@@ -448,17 +443,14 @@ int hexo_autotune(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, uint32_t 
                     if (r > 0) total += bt->path_ms;
                 }
                 (void)hipStreamSynchronize(ctx->stream);
-                if (rc != HEXO_OK) {
-                    ctx->tuning = saved;
-                    return ctx->set_error(rc, err);
-                }
+                if (rc != HEXO_OK) return ctx->set_error(rc, err);
                 const double ms = total / reps;
                 if (ms < best_ms) {
                     best_ms = ms;
-                    winner = ctx->tuning;
+                    winner = trial;
                 }
             }
-    ctx->tuning = winner;
+    ctx->set_tuning(winner);
     if (best) *best = winner;
     return HEXO_OK;
 }

@@ -313,11 +313,13 @@ PinnedBuf::~PinnedBuf() {
 }
 
 int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_t stream, Arena* arena,
-                 std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n, PinnedBuf* stage) {
+                 std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n, PinnedBuf* stage,
+                 const hexo_tuning* tuning) {
     if (n == 0 || !reqs) {
         err = "empty batch";
         return HEXO_ERR_INVALID;
     }
+    const hexo_tuning tun = tuning ? *tuning : ctx->tuning_now();
     std::vector<ReqPlan> plans(n);
     uint64_t n_seg = 0, n_fin = 0, n_opt = 0, n_partials = 0, n_groups = 0, n_out = 0;
     uint64_t n_geo_items = 0, n_geo_pts = 0, n_geo_waves = 0;
@@ -354,7 +356,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     b->n_req = n;
     b->n_out_total = n_out;
     b->ppt[0] = 1;
-    for (uint32_t cls = 1; cls < kPathClasses; ++cls) b->ppt[cls] = ctx->tuning.paths_per_thread;
+    for (uint32_t cls = 1; cls < kPathClasses; ++cls) b->ppt[cls] = tun.paths_per_thread;
     const uint64_t n_finals = finals_n ? *finals_n : 0;
     const Layout L =
         make_layout(n, n_seg, n_fin, n_opt, n_cls, n_partials, n_groups, n_out, n_finals, n_geo_items, n_geo_pts,
@@ -615,7 +617,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     for (uint32_t cls = 0; cls < kPathClasses; ++cls) {
         if (!b->has_class[cls]) continue;
         const uint32_t prec = class_precision(cls);
-        const uint32_t block = cls == 0 ? 256u : ctx->tuning.block_threads;
+        const uint32_t block = cls == 0 ? 256u : tun.block_threads;
         const size_t lds = path_lds_bytes(prec, b->ppt[cls], block);
         int bpc = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc,
@@ -624,8 +626,8 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
                                                           (int)block, lds) != hipSuccess ||
             bpc <= 0)
             bpc = std::max(1, 1024 / (int)block);
-        if (cls != 0 && ctx->tuning.waves_per_simd)
-            bpc = std::max(1, std::min(bpc, (int)(ctx->tuning.waves_per_simd * 4 * 64 / block)));
+        if (cls != 0 && tun.waves_per_simd)
+            bpc = std::max(1, std::min(bpc, (int)(tun.waves_per_simd * 4 * 64 / block)));
         const uint64_t waves_per_block = block / kTile;
         const uint64_t need = (b->pa[cls].total_items + waves_per_block - 1) / waves_per_block;
         b->grid[cls] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, (uint64_t)n_cu * (uint64_t)bpc));

@@ -147,7 +147,8 @@ struct hexo_ctx {
     int device = 0;
     hipDeviceProp_t prop{};
     hipStream_t stream = nullptr;
-    hexo_tuning tuning{2, 256, 0, 0};
+    hexo_tuning tuning{2, 256, 0, 0};  // guarded by tuning_mu: the serving thread reads it too
+    std::mutex tuning_mu;
     std::recursive_mutex mu;
     std::mutex err_mu;  // guards last_error: failures may be reported from several threads at once
     std::string last_error;
@@ -166,6 +167,15 @@ struct hexo_ctx {
         last_error = msg;
         return code;
     }
+    hexo_tuning tuning_now() {
+        std::lock_guard<std::mutex> g(tuning_mu);
+        return tuning;
+    }
+    void set_tuning(const hexo_tuning& t) {
+        std::lock_guard<std::mutex> g(tuning_mu);
+        tuning = t;
+        tuning.reserved = 0;
+    }
     std::shared_ptr<struct hexo_server_state> server_ref() {
         std::lock_guard<std::mutex> g(server_mu);
         return server;
@@ -180,9 +190,10 @@ namespace hexo {
 
 // Creates a batch: validates, builds the schedules, packs and uploads the tables, plans the
 // launches. mem/arena: when arena is non-null the device region comes from it.
+// tuning: the launch shape to plan with (null: the context's, read once under tuning_mu).
 int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_t stream, Arena* arena,
                  std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n = nullptr,
-                 PinnedBuf* stage = nullptr);
+                 PinnedBuf* stage = nullptr, const hexo_tuning* tuning = nullptr);
 // Device bytes a set of requests needs (admission accounting).
 size_t batch_device_bytes(const hexo_request* reqs, uint32_t n);
 size_t request_device_bytes(const hexo_request* r);
