@@ -298,6 +298,9 @@ int hexo_paths(hexo_ctx* ctx, const hexo_request* req, uint64_t first, uint64_t 
     sub.path_offset = req->path_offset + (first - lead);
     sub.n_paths = n + lead;
     const uint64_t n_fin = sub.n_paths * (uint64_t)req->n_expiries * 2;
+    // the NaN fill below counts 32-bit words in a uint32_t
+    if (n_fin > (1ull << 30))
+        return ctx->set_error(HEXO_ERR_INVALID, "at most 2^30 finals (paths x expiries x 2) per hexo_paths call");
     std::string err;
     std::unique_ptr<hexo::Batch> b;
     int rc = hexo::batch_create(ctx, &sub, 1, ctx->stream, nullptr, b, err, &n_fin);

@@ -185,7 +185,8 @@ int hexo_geo_asian_price(hexo_ctx* ctx, const hexo_request* req, double* prices)
  * hexo_paths: per-path (Asian average A_c, interpolated X_T,c) at every expiry, for paths
  * [first, first+n) of the request, finals[(p*n_expiries + c)*2 + {0,1}], in the request's
  * precision (HQEAnderson::operator++ and the policies' accumulate_final_value,
- * src/HSimulation.tpp:52-86, src/inc/AsianContract.h:29-34, src/inc/VanillaContract.h:28-31). */
+ * src/HSimulation.tpp:52-86, src/inc/AsianContract.h:29-34, src/inc/VanillaContract.h:28-31).
+ * At most 2^26 paths and 2^30 finals per call; expiries never finalised read NaN. */
 int hexo_paths(hexo_ctx* ctx, const hexo_request* req, uint64_t first, uint64_t n, double* finals);
 /* Raw stream words, computed on the GPU: out[i*n_words + j] = word word0+j of path path0+i
  * (replaces the uniform ring refill of RNG::setup_u, src/RNG.cpp:28-33). */

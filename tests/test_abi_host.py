@@ -200,3 +200,4 @@ def test_asian_weights_refuse_before_allocating():
     # the grid reaches T = 1 exactly (2^22 steps of 2^-22): the reference's final term replaces the
     # last trapezoid, so the weights sum to 1 - 1/N
     assert abs(ww.sum() - (1.0 - 2.0 ** -22)) < 1e-9
+

@@ -7,6 +7,7 @@
 (3) The largest step count per expiry (2^22), FP32 with the martingale correction: at K = 0 the
     Asian and European prices are E[A] = sum_j w_j S0 e^{r t_j} and E[X_T] = S0 e^{r T}, met within
     5 standard errors.
+(4) hexo_paths refuses windows beyond its limits before it allocates or writes.
 """
 import dataclasses
 
@@ -96,3 +97,21 @@ def test_largest_step_count(ctx):
     assert np.all(np.abs(p - expect) < 5 * s)
     req.steps = 2 ** 22 + 1
     assert H.validate(req) == -1
+
+
+def test_paths_window_limits(ctx):
+    """hexo_paths refuses windows beyond 2^26 paths or 2^30 finals (paths x expiries x 2) before it
+    allocates or writes anything (the output buffer here holds one path's 17 x 2 finals)."""
+    import ctypes as C
+    from hestonexotics_amd import _lib
+    lib = _lib.load()
+    buf = np.zeros(17 * 2)
+    out = buf.ctypes.data_as(C.POINTER(C.c_double))
+    T = [0.1 * (i + 1) for i in range(17)]
+    req = H.Request(**BASE, expiries=T, strikes=[[100.0]] * 17, steps=4, n_paths=2 ** 26 + 64, seed=1,
+                    precision=H.FP32)
+    q = req.to_c()
+    assert lib.hexo_paths(ctx.h, C.byref(q), 0, 2 ** 26 + 1, out) == -1      # too many paths
+    assert lib.hexo_paths(ctx.h, C.byref(q), 0, 2 ** 25, out) == -1          # 2^25 x 17 x 2 > 2^30 finals
+    assert "2^30 finals" in lib.hexo_ctx_last_error(ctx.h).decode()
+    assert lib.hexo_paths(ctx.h, C.byref(q), 0, 1, out) == 0 and np.all(np.isfinite(buf))
