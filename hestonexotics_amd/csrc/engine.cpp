@@ -288,6 +288,13 @@ size_t request_device_bytes(const hexo_request* r) {
            p.geo.items.size() * (sizeof(DevGeo) + 4 + (kGeoMaxP0 + 1) * 520 + 8 * 67) + p.geo.pts.size() * sizeof(double);
 }
 
+double request_work(const hexo_request* r) {
+    ReqPlan p;
+    if (!plan_request(r, p)) return 0.0;
+    // the plain FP64 kernel takes about seven times as long per path-step as the FP32 ones
+    return (double)r->n_paths * (double)p.total_steps * (r->precision == HEXO_FP64 ? 7.0 : 1.0);
+}
+
 size_t batch_device_bytes(const hexo_request* reqs, uint32_t n) {
     size_t s = 40 * 256 + 128 * sizeof(double);  // alignment slack of the region table, quadrature table
     for (uint32_t i = 0; i < n; ++i) s += request_device_bytes(&reqs[i]);

@@ -197,6 +197,8 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
 // Device bytes a set of requests needs (admission accounting).
 size_t batch_device_bytes(const hexo_request* reqs, uint32_t n);
 size_t request_device_bytes(const hexo_request* r);
+// Simulated path-steps of a request in FP32-kernel units (scheduling weight; 0 if invalid).
+double request_work(const hexo_request* r);
 // Enqueue the batch's launches (eager) on its stream; timed = record HIP events around them.
 int batch_enqueue(Batch& b, PathMode mode, bool timed, double* d_finals, std::string& err);
 int batch_capture_graph(Batch& b, std::string& err);

@@ -7,11 +7,25 @@
 // a stream slot free up, so their work overlaps the work already on the device. Results are the
 // same as pricing each request alone: a request's tiles, reduction order and RNG stream depend
 // only on the request.
+//
+// Large and small requests are kept apart, because a batch's results are ready when its
+// longest request is. A request above kLargeWork path-steps runs in a batch of its own, one such
+// batch at a time, with its grid capped at kLargeWavesPerSimd waves per SIMD (the FP32 kernels
+// hold at most 5). The free wave slot on every SIMD lets a small batch launched later run beside
+// it at once, instead of waiting for the persistent grid to drain. Small requests are batched in
+// arrival order up to kSmallBatchWork path-steps, on the other streams. A large request waiting
+// for its turn does not hold up the small ones behind it.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 
 #include "engine.hpp"
+
+namespace {
+constexpr double kLargeWork = 1e9;          // FP32 path-steps: about 2 ms alone on the GPU
+constexpr double kSmallBatchWork = 5e8;     // work of one batch of small requests
+constexpr uint32_t kLargeWavesPerSimd = 4;  // grid cap of a large request's batch
+}  // namespace
 
 struct hexo_server_state {
     struct Ticket {
@@ -23,6 +37,8 @@ struct hexo_server_state {
         int state = 0;  // 0 queued, 1 in flight, 2 done, 3 failed/cancelled
         size_t bytes = 0;
         uint64_t n_out = 0;
+        double work = 0.0;  // request_work
+        bool large = false;
     };
     struct Flight {
         std::unique_ptr<hexo::Batch> b;
@@ -31,6 +47,7 @@ struct hexo_server_state {
         int slot = 0;
         uint32_t batch_id = 0;
         size_t bytes = 0;
+        bool large = false;
     };
 
     hexo_ctx* ctx = nullptr;
@@ -114,20 +131,44 @@ void hexo_server_state::launch_ready() {
                 break;
             }
         if (slot < 0) return;
-        // admit the head of the queue, in order, while the budget has room
         std::vector<uint64_t> ids;
         std::vector<hexo_request> reqs;
         size_t bytes = 16 * 256;
         const uint32_t max_batch = cfg.max_batch ? cfg.max_batch : 4096;
-        while (!queue.empty() && ids.size() < max_batch) {
-            Ticket& t = *tickets[queue.front()];
-            if (reserved + bytes + t.bytes > budget) break;
-            bytes += t.bytes;
+        uint32_t large_in_flight = 0;
+        for (const Flight& f : flights) large_in_flight += f.large ? 1u : 0u;
+        const bool large_slot = large_in_flight == 0;
+        bool large = false;
+        // a large request at the head runs alone, on a large slot, once the budget has room
+        Ticket& head = *tickets[queue.front()];
+        if (head.large && large_slot) {
+            if (reserved + bytes + head.bytes > budget) return;  // wait for memory to free up
+            bytes += head.bytes;
             ids.push_back(queue.front());
-            reqs.push_back(t.req);
+            reqs.push_back(head.req);
             queue.pop_front();
+            large = true;
         }
-        if (ids.empty()) return;  // wait for memory to free up
+        if (ids.empty()) {
+            // small requests in arrival order, past large ones waiting for a large slot (they keep
+            // their place), while the budget has room
+            double work = 0.0;
+            for (auto it = queue.begin(); it != queue.end() && ids.size() < max_batch;) {
+                Ticket& t = *tickets[*it];
+                if (t.large) {
+                    ++it;
+                    continue;
+                }
+                if (reserved + bytes + t.bytes > budget) break;
+                if (!ids.empty() && work + t.work > kSmallBatchWork) break;
+                bytes += t.bytes;
+                work += t.work;
+                ids.push_back(*it);
+                reqs.push_back(t.req);
+                it = queue.erase(it);
+            }
+        }
+        if (ids.empty()) return;  // wait for memory or a slot to free up
         const double t_admit = hexo::now_s();
         if (inject_admit) {
             --inject_admit;
@@ -138,11 +179,15 @@ void hexo_server_state::launch_ready() {
         }
         std::unique_ptr<hexo::Batch> b;
         std::string err;
+        hexo_tuning tun = ctx->tuning_now();
+        if (large && (tun.waves_per_simd == 0 || tun.waves_per_simd > kLargeWavesPerSimd))
+            tun.waves_per_simd = kLargeWavesPerSimd;
         int rc = hexo::batch_create(ctx, reqs.data(), (uint32_t)reqs.size(), streams[slot], &arena, b, err, nullptr,
-                                    stages[slot].get());
+                                    stages[slot].get(), &tun);
         if (rc == HEXO_ERR_NOMEM && !flights.empty()) {
-            // fragmentation: put the requests back and retry when a batch completes
-            for (auto it = ids.rbegin(); it != ids.rend(); ++it) queue.push_front(*it);
+            // fragmentation: put the requests back in arrival order and retry when a batch completes
+            for (uint64_t id : ids) queue.push_back(id);
+            std::sort(queue.begin(), queue.end());  // ticket ids grow with arrival
             return;
         }
         if (rc == HEXO_OK && inject_launch) {
@@ -186,6 +231,7 @@ void hexo_server_state::launch_ready() {
         f.slot = slot;
         f.batch_id = batch_counter;
         f.bytes = bytes;
+        f.large = large;
         flights.push_back(std::move(f));
     }
 }
@@ -290,6 +336,8 @@ int hexo_submit(hexo_ctx* ctx, const hexo_request* req, uint64_t* ticket) {
     t->req.strikes = t->K.data();
     t->n_out = n_opts * hexo::n_payoffs(req->payoffs);
     t->bytes = hexo::request_device_bytes(req) + 4 * 256;
+    t->work = hexo::request_work(req);
+    t->large = t->work > kLargeWork;
     if (t->bytes + 16 * 256 > s->budget) {
         s->st.rejected++;
         return ctx->set_error(HEXO_ERR_ADMISSION, "request needs " + std::to_string(t->bytes) +

@@ -212,7 +212,10 @@ int hexo_copy_probe(hexo_ctx* ctx, uint64_t bytes, uint32_t reps, double* ms_per
 
 /* ---- serving: continuous batching ----
  * Requests are admitted by device-memory budget as they arrive; admitted requests are
- * launched in batches on up to `streams` HIP streams, so new work overlaps work in flight. */
+ * launched in batches on up to `streams` HIP streams, so new work overlaps work in flight.
+ * A request above 1e9 path-steps runs in a batch of its own, one such batch at a time and on at
+ * most 4 waves per SIMD; smaller requests are batched together beside it and may pass a large
+ * request that waits for its turn. */
 typedef struct hexo_server_config {
     uint64_t memory_budget_bytes; /* 0 = a fraction of free device memory */
     uint32_t streams;             /* concurrent in-flight batches, 1..4 */
