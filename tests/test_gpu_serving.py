@@ -3,6 +3,8 @@
 Each request's result is the same alone (hexo_price) and in the mix (served in whatever batch it
 landed in), bit for bit: its tiles, reduction order and RNG stream depend only on the request.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -177,3 +179,29 @@ def test_stop_while_clients_submit_and_wait(ctx):
     assert all(r[1] in (0, -9) for r in results), sorted(set(r[1] for r in results))
     with pytest.raises(H.HexoError):
         ctx.submit(base)
+
+
+def test_small_requests_pass_a_large_one(ctx):
+    """A large request (8e9 path-steps, about 18 ms alone) submitted first runs in a batch of its
+    own on part of the GPU; tiny requests submitted right after it run beside it and finish first
+    (sharing its batch, they would finish with it). Results equal the requests priced alone."""
+    say("gpu serving: small requests pass a large one")
+    big = H.Request(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0, expiries=[2.0],
+                    strikes=[[100.0]], steps=4000, n_paths=2_000_000, seed=3, precision=H.FP32)
+    tiny = [dataclasses.replace(big, steps=10, n_paths=640, seed=10 + i) for i in range(6)]
+    ctx.server_start(streams=2)
+    try:
+        tb = ctx.submit(big)
+        tt = [ctx.submit(r) for r in tiny]
+        out_t = [ctx.wait(t, r.n_outputs) for t, r in zip(tt, tiny)]
+        p_big, _, info_big = ctx.wait(tb, big.n_outputs)
+    finally:
+        ctx.server_stop()
+    done_tiny = max(i["done_s"] for _, _, i in out_t)
+    say(f"  large done after {1e3 * (info_big['done_s'] - info_big['arrival_s']):.1f} ms, tiny after "
+        f"{1e3 * (done_tiny - info_big['arrival_s']):.1f} ms; large batch size {info_big['batch_size']}")
+    assert info_big["batch_size"] == 1
+    assert done_tiny < info_big["done_s"]
+    for r, (p, _, _) in zip(tiny, out_t):
+        np.testing.assert_array_equal(p, ctx.price(r)[0])
+    np.testing.assert_array_equal(p_big, ctx.price(big)[0])
