@@ -200,29 +200,40 @@ int price_all(const std::string& sym, const std::string& vol_type, long vol_n) {
 }
 
 // Moments of 2^26 uniforms and normals drawn on the GPU (src/UnitTest.cpp:528-564 draws 2^28).
+// The reference's rng_test (src/UnitTest.cpp:528-564) at its sample size: 2^28 uniforms and 2^28
+// normals (the FP32 path's PPND7) from the Philox stream, mean and variance of each against the
+// exact moments. The reference's bound 1e-4 is kept where it is at least five standard errors at
+// this size (the uniform's mean, 5.5 SE, and variance, 22 SE); the normal's mean and variance
+// have standard errors 6.1e-5 and 8.6e-5, so they are held to five of those.
 int test_rng() {
     hexo_ctx* ctx = HSimulation::detail::context();
-    const uint64_t n_paths = 1 << 14, n_words = 1 << 12;
-    std::vector<uint32_t> w(n_paths * n_words);
-    if (hexo_rng_fill(ctx, 1, 0, 0, n_paths, 0, n_words, w.data()) != HEXO_OK) return 1;
-    double s = 0, s2 = 0;
-    for (uint32_t k : w) {
-        const double u = (k + 0.5) * 0x1p-32;
-        s += u;
-        s2 += u * u;
+    const uint64_t chunk_paths = 1 << 12, chunk_words = 1 << 12, n_chunks = 16;  // 16 x 2^24 = 2^28
+    std::vector<uint32_t> w(chunk_paths * chunk_words);
+    std::vector<double> z(w.size());
+    double s = 0, s2 = 0, zs = 0, zs2 = 0;
+    for (uint64_t c = 0; c < n_chunks; ++c) {
+        if (hexo_rng_fill(ctx, 1, 0, c * chunk_paths, chunk_paths, 0, chunk_words, w.data()) != HEXO_OK ||
+            hexo_normals(ctx, w.data(), w.size(), HEXO_FP32, z.data()) != HEXO_OK)
+            return 1;
+        double cs = 0, cs2 = 0, czs = 0, czs2 = 0;  // per-chunk sums, then added: short summation chains
+        for (size_t i = 0; i < w.size(); ++i) {
+            const double u = (w[i] + 0.5) * 0x1p-32;
+            cs += u;
+            cs2 += u * u;
+            czs += z[i];
+            czs2 += z[i] * z[i];
+        }
+        s += cs;
+        s2 += cs2;
+        zs += czs;
+        zs2 += czs2;
     }
-    const double n = (double)w.size(), mu = s / n, var = s2 / n - mu * mu;
-    std::vector<double> z(1 << 22);
-    if (hexo_normals(ctx, w.data(), z.size(), HEXO_FP32, z.data()) != HEXO_OK) return 1;
-    double zs = 0, zs2 = 0;
-    for (double x : z) {
-        zs += x;
-        zs2 += x * x;
-    }
-    const double zn = (double)z.size(), zm = zs / zn, zv = zs2 / zn - zm * zm;
-    std::printf("uniform mean %.6f var %.6f; normal mean %.6f var %.6f\n", mu, var, zm, zv);
-    const bool ok = std::fabs(mu - 0.5) < 1e-4 && std::fabs(var - 1.0 / 12) < 1e-4 && std::fabs(zm) < 2e-3 &&
-                    std::fabs(zv - 1.0) < 4e-3;
+    const double n = (double)(n_chunks * w.size());
+    const double mu = s / n, var = (s2 - n * mu * mu) / (n - 1);
+    const double zm = zs / n, zv = (zs2 - n * zm * zm) / (n - 1);
+    std::printf("sample size %.0f: uniform mean %.7f var %.7f; normal mean %.7f var %.7f\n", n, mu, var, zm, zv);
+    const bool ok = std::fabs(mu - 0.5) < 1e-4 && std::fabs(var - 1.0 / 12) < 1e-4 &&
+                    std::fabs(zm) < 5.0 / std::sqrt(n) && std::fabs(zv - 1.0) < 5.0 * std::sqrt(2.0 / n);
     std::printf("RNG test %s\n", ok ? "passed" : "FAILED");
     return ok ? 0 : 1;
 }

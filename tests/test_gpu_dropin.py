@@ -41,8 +41,11 @@ def test_cli_prices_equal_c_abi(ctx):
 
 
 def test_cli_self_tests():
+    """`hexo -t rng` is the reference's rng_test at its sample size (2^28 uniforms and normals);
+    `hexo -t simulation` prices one Asian call with both kernels."""
     print("gpu cli -t rng -t simulation", flush=True)
     out = run_cli("-t", "rng", "-t", "simulation")
+    print(out.stdout, flush=True)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "RNG test passed" in out.stdout and "Simulation test passed" in out.stdout
 
