@@ -20,7 +20,8 @@
 // Chains with T <= 0.1 years are dropped and calls are kept when the selected volume field is
 // >= <# volume> (src/WebAPI.cpp:62-84,101-105). The output line per option has the reference's
 // columns (src/Main.cpp:89-93). Environment: HEXO_PATHS (1e5), HEXO_STEPS (1e3),
-// HEXO_PRECISION (fp64 | fp32, default fp64), HEXO_SEED (1).
+// HEXO_PRECISION (fp64 | fp32, default fp64), HEXO_SEED (1); variance reduction, off by default:
+// HEXO_CONTROL_VARIATE (none | geo | regression), HEXO_MARTINGALE (0 | 1).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -172,6 +173,23 @@ std::vector<ffloat> run_price(const UnderlyingData& d, unsigned paths, unsigned 
     return HSimulation::price<Scheme>(d.p, d.S, d.chains, paths, n_opts, steps);
 }
 
+// The scheme from the environment: precision, Andersen's martingale correction, and the
+// geometric-average control variate (coefficient 1 or the regression coefficient).
+template <typename acc, class Policy>
+std::vector<ffloat> run_scheme(bool mart, const UnderlyingData& d, unsigned paths, unsigned n_opts, unsigned steps) {
+    return mart ? run_price<HSimulation::HQEAndersonMartingale<acc, Policy>>(d, paths, n_opts, steps)
+                : run_price<HSimulation::HQEAnderson<acc, Policy>>(d, paths, n_opts, steps);
+}
+
+template <typename acc>
+std::vector<ffloat> run_policy(const std::string& cv, bool mart, const UnderlyingData& d, unsigned paths,
+                               unsigned n_opts, unsigned steps) {
+    if (cv == "geo") return run_scheme<acc, AAsianCallGeometricCV>(mart, d, paths, n_opts, steps);
+    if (cv == "regression") return run_scheme<acc, AAsianCallGeometricCVRegression>(mart, d, paths, n_opts, steps);
+    if (!cv.empty() && cv != "none") throw std::runtime_error("HEXO_CONTROL_VARIATE must be none, geo or regression");
+    return run_scheme<acc, AAsianCallNonAdaptive>(mart, d, paths, n_opts, steps);
+}
+
 int price_all(const std::string& sym, const std::string& vol_type, long vol_n) {
     UnderlyingData d = read_chains(sym, vol_type, vol_n);
     if (!d.have_params && !hexo_cli::fetch_params(data_dir(), sym, d.p)) d.p = calibrate_and_store(sym, d);
@@ -184,9 +202,11 @@ int price_all(const std::string& sym, const std::string& vol_type, long vol_n) {
     HSimulation::set_seed(env_u("HEXO_SEED", 1));
     const char* prec = std::getenv("HEXO_PRECISION");
     const bool fp32 = prec && std::string(prec) == "fp32";
-    std::vector<ffloat> res =
-        fp32 ? run_price<HSimulation::HQEAnderson<float, AAsianCallNonAdaptive>>(d, paths, n_opts, steps)
-             : run_price<HSimulation::HQEAnderson<ffloat, AAsianCallNonAdaptive>>(d, paths, n_opts, steps);
+    const char* cv_env = std::getenv("HEXO_CONTROL_VARIATE");
+    const std::string cv = cv_env ? cv_env : "";
+    const bool mart = env_u("HEXO_MARTINGALE", 0) != 0;
+    std::vector<ffloat> res = fp32 ? run_policy<float>(cv, mart, d, paths, n_opts, steps)
+                                   : run_policy<ffloat>(cv, mart, d, paths, n_opts, steps);
     unsigned i = 0;
     for (const options_chain& c : d.chains)
         for (const option& o : c.options)

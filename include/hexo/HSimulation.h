@@ -13,6 +13,14 @@
  *   OptionPolicy EuropeanCallNonAdaptive -> European calls (src/inc/VanillaContract.h:14-40)
  *   accumulate_t double -> the plain FP64 kernel (HEXO_FP64); float -> FP32 stepping with FP64
  *   reduction (HEXO_FP32).
+ * Variance reduction, which the reference suggests beside price (src/inc/HSimulation.h:48-60),
+ * as further policies and a scheme (extensions, not in the reference):
+ *   OptionPolicy AAsianCallGeometricCV           -> Asian calls with the geometric-average control
+ *                                                   variate, coefficient 1 (HEXO_ASIAN_GEO_CV)
+ *   OptionPolicy AAsianCallGeometricCVRegression -> the same with the regression coefficient
+ *                                                   (HEXO_CV_REGRESSION)
+ *   Scheme HQEAndersonMartingale<accumulate_t, OptionPolicy> -> Andersen's martingale-corrected
+ *                                                   step (HEXO_MARTINGALE)
  * The estimator is the reference's: undiscounted, no drift term, its time grid and Asian
  * finalisation; outputs in chain-major option order (src/HSimulation.tpp:40). Differences: the
  * paths come from a counter-based Philox4x32-10 stream (seed set with HSimulation::set_seed, 1 by
@@ -35,12 +43,27 @@
 #include "Types.h"
 #endif
 
-// Option policies of the reference, as tags (their accumulation logic runs in the kernel).
+// Option policies of the reference, as tags (their accumulation logic runs in the kernel):
+// the payoff blocks to price, flags, and which block (in payoff bit order) price returns.
 struct AAsianCallNonAdaptive {
     static constexpr uint32_t hexo_payoff = HEXO_ASIAN;
+    static constexpr uint32_t hexo_flags = 0;
+    static constexpr uint32_t hexo_block = 0;
 };
 struct EuropeanCallNonAdaptive {
     static constexpr uint32_t hexo_payoff = HEXO_EUROPEAN;
+    static constexpr uint32_t hexo_flags = 0;
+    static constexpr uint32_t hexo_block = 0;
+};
+struct AAsianCallGeometricCV {
+    static constexpr uint32_t hexo_payoff = HEXO_ASIAN_GEO_CV;
+    static constexpr uint32_t hexo_flags = 0;
+    static constexpr uint32_t hexo_block = 0;
+};
+struct AAsianCallGeometricCVRegression {  // blocks: Asian, control variate, geometric
+    static constexpr uint32_t hexo_payoff = HEXO_ASIAN | HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN;
+    static constexpr uint32_t hexo_flags = HEXO_CV_REGRESSION;
+    static constexpr uint32_t hexo_block = 1;
 };
 
 namespace HSimulation {
@@ -51,7 +74,16 @@ template <typename accumulate_t, class OptionPolicy>
 class HQEAnderson {
    public:
     static constexpr uint32_t payoff = OptionPolicy::hexo_payoff;
+    static constexpr uint32_t flags = OptionPolicy::hexo_flags;
+    static constexpr uint32_t block = OptionPolicy::hexo_block;
     static constexpr uint32_t precision = sizeof(accumulate_t) == sizeof(double) ? HEXO_FP64 : HEXO_FP32;
+};
+
+// The QE scheme with Andersen's martingale correction of K0 (HEXO_MARTINGALE).
+template <typename accumulate_t, class OptionPolicy>
+class HQEAndersonMartingale : public HQEAnderson<accumulate_t, OptionPolicy> {
+   public:
+    static constexpr uint32_t flags = OptionPolicy::hexo_flags | HEXO_MARTINGALE;
 };
 
 namespace detail {
@@ -116,15 +148,17 @@ std::vector<ffloat> price_with_errors(const HParams& p, const ffloat S, const st
     r.n_strikes = ns.data();
     r.strikes = K.data();
     r.payoffs = Scheme::payoff;
-    r.flags = 0;
+    r.flags = Scheme::flags;
     r.precision = Scheme::precision;
-    std::vector<ffloat> prices(n_opts), ses(n_opts);
+    const uint64_t n_out = hexo_request_n_outputs(&r);
+    std::vector<double> prices(n_out ? n_out : n_opts), ses(prices.size());
     hexo_ctx* ctx = detail::context();
     const int rc = hexo_price(ctx, &r, prices.data(), ses.data());
     if (rc != HEXO_OK)
         throw std::runtime_error(std::string("hexo: ") + hexo_strerror(rc) + ": " + hexo_ctx_last_error(ctx));
-    if (std_errs) *std_errs = std::move(ses);
-    return prices;
+    const size_t b0 = (size_t)Scheme::block * n_opts;
+    if (std_errs) std_errs->assign(ses.begin() + b0, ses.begin() + b0 + n_opts);
+    return std::vector<ffloat>(prices.begin() + b0, prices.begin() + b0 + n_opts);
 }
 
 // src/inc/HSimulation.h:61-63

@@ -40,6 +40,33 @@ def test_cli_prices_equal_c_abi(ctx):
     assert out.returncode == 0 and len([l for l in out.stdout.splitlines() if l.startswith("S:")]) == 3
 
 
+@pytest.mark.parametrize("cv,mart,prec", [("geo", "0", "fp64"), ("regression", "1", "fp32"), ("none", "1", "fp64")])
+def test_cli_variance_reduction_equals_c_abi(ctx, cv, mart, prec):
+    """The wrapper's variance-reduction policies and martingale-corrected scheme, selected in the CLI
+    with HEXO_CONTROL_VARIATE and HEXO_MARTINGALE, give the C ABI's prices of the same request: the
+    control-variate block (coefficient 1 or the regression coefficient) or the corrected Asian."""
+    print("gpu cli variance reduction", cv, mart, prec, flush=True)
+    out = run_cli("-p", "asian", "all", "DEMO",
+                  env_extra={"HEXO_PATHS": "20000", "HEXO_STEPS": "200", "HEXO_CONTROL_VARIATE": cv,
+                             "HEXO_MARTINGALE": mart, "HEXO_PRECISION": prec})
+    assert out.returncode == 0, out.stderr
+    got = [float(re.search(r"asian-option-price: ([-0-9.]+)", l).group(1))
+           for l in out.stdout.splitlines() if l.startswith("S:")]
+    payoffs, flags, block = {"geo": (H.ASIAN_GEO_CV, 0, 0),
+                             "regression": (H.ASIAN | H.ASIAN_GEO_CV | H.GEO_ASIAN, H.CV_REGRESSION, 1),
+                             "none": (H.ASIAN, 0, 0)}[cv]
+    req = H.Request(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0, expiries=[65 / 261, 1.0],
+                    strikes=[[90, 100, 110], [100, 120]], steps=200, n_paths=20000, seed=1, payoffs=payoffs,
+                    flags=flags | (H.MARTINGALE if mart == "1" else 0),
+                    precision=H.FP32 if prec == "fp32" else H.FP64)
+    p, _ = ctx.price(req)
+    n = req.n_opts
+    assert len(got) == n
+    np.testing.assert_allclose(got, np.round(p[block * n:(block + 1) * n], 2), atol=0.0051)
+    bad = run_cli("-p", "asian", "all", "DEMO", env_extra={"HEXO_CONTROL_VARIATE": "antithetic"})
+    assert bad.returncode == 2
+
+
 def test_cli_self_tests():
     """`hexo -t rng` is the reference's rng_test at its sample size (2^28 uniforms and normals);
     `hexo -t simulation` prices one Asian call with both kernels."""
