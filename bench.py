@@ -31,9 +31,10 @@ PEAK_FP64_TFLOPS = 78.6
 
 # Algorithmic FP32 flops of the path kernel (kernels.hip), convention: add/sub/mul = 1, fma = 2,
 # exp2/log2/sqrt/rcp = 1; compares, selects, conversions and integer (RNG) work are not counted.
-F_STEP = 22      # moments m, s2, m^2, 1.5 m^2 (7); log-price step incl. exp2 (14); Asian sum (1)
-F_QUAD = 20      # quadratic variance branch without its normal (incl. a Newton-refined reciprocal)
-F_EXP = 20       # exponential variance branch incl. its uniform (two Newton-refined reciprocals)
+F_STEP = 24      # moments th - V, m, s2, m^2, 1.5 m^2 (7); V', V' - V, V + V' - 2 th and the log-price
+                 # step incl. exp2 (16); Asian sum (1)
+F_QUAD = 17      # quadratic variance branch without its normal: 2/psi, b^2, b, a, V' - m = a (Z (Z + 2b) - 1)
+F_EXP = 21       # exponential variance branch incl. its uniform (two Newton-refined reciprocals), V' - m
 F_CENTRAL = 19   # one AS241 PPND7 normal, central region (q, r, 3+3 FMA rational, rcp, 2 mul)
 F_TAIL = 20      # one AS241 PPND7 normal, tail region (argument, log, sqrt, 3+2 FMA rational, rcp)
 F_SEG = 8        # per path and run of steps: Asian run sum folding and the new-dt accumulation

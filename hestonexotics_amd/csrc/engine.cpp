@@ -123,6 +123,15 @@ std::string validate(const hexo_request* r) {
     uint64_t total = 0;
     if (!build_schedule(r->expiries, r->n_expiries, r->n_strikes, r->steps, segs, total))
         return "time grid could not be built";
+    if (r->precision != HEXO_FP64) {
+        // the FP32 step's moment ratio 2 m^2 / s2 ~ 2 V / (sigma^2 dt) must stay below sqrt(FLT_MAX)
+        // (1.8e19). This bound leaves a factor of 9 for V above max(v0, theta); it only binds for
+        // tiny sigma, where V hardly leaves its deterministic path between v0 and theta.
+        double dmin = 1e300;
+        for (const HostSeg& S : segs) dmin = std::min(dmin, S.delta);
+        if (r->sigma * r->sigma * dmin < 1e-18 * std::max(r->v0, r->theta))
+            return "sigma^2 dt below 1e-18 max(v0, theta) overflows the FP32 variance step; use HEXO_FP64";
+    }
     if (r->payoffs & (HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN)) {
         GeoPlan g;
         std::string why;
@@ -455,14 +464,18 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
             const double dl = S.delta, k = q.kappa, th = q.theta, e = q.sigma, rho = q.rho;
             const double dd = std::exp(-k * dl), omd = -std::expm1(-k * dl);
             g.omd = (float)omd;
+            g.ope = (float)(1.0 + dd);
             g.th = (float)th;
             g.c1 = (float)(e * e * dd * omd / k);
             g.c2 = (float)(th * e * e * omd * omd / (2.0 * k));
-            const double K0 = -rho * k * th / e * dl + d.r_drift * dl;
             const double kc = 0.5 * dl * (k * rho / e - 0.5);
             const double kd = rho / e;
             const double K3 = 0.5 * dl * (1.0 - rho * rho);
-            g.g0 = (float)((K0 + 2.0 * (double)g.th * kc) * L2E);
+            // g0 = K0 + 2 th kc (src/HSimulation.tpp:75-80 with the drift extension), taken with the
+            // FP32 level th_f in both terms since the FP32 variance reverts to th_f: the rho/sigma
+            // parts cancel exactly, leaving (r - th_f/2) dt. With the exact theta in K0 they would
+            // leave rho kappa dt (th_f - theta) / sigma per step, 1e-5 at sigma = 1e-6.
+            g.g0 = (float)((d.r_drift * dl - 0.5 * (double)g.th * dl) * L2E);
             g.kc = (float)(kc * L2E);
             g.kd = (float)(kd * L2E);
             g.k3 = (float)(K3 * L2E * L2E);

@@ -249,6 +249,7 @@ __device__ __forceinline__ void count_flush(unsigned long long* counts, uint32_t
 struct SegF {
     float omd, th, c1, c2, g0, kc, kd, k3;
     float mA, ma, mh, mr;
+    float ope;
 };
 
 // Per-wave LDS scratch of the FP32 kernel: the tail-normal queue during the steps (2*PPT words in,
@@ -295,14 +296,17 @@ struct StepCount {
     uint32_t quad = 0, exp = 0, central = 0, tail = 0;
 };
 
-// The step's conditional moments from V (src/HSimulation.tpp:58-62): m, s2, m^2 and the branch
-// (quadratic when psi = s2/m^2 < 1.5).
+// The step's conditional moments from V (src/HSimulation.tpp:58-62): d0 = th - V, m = V + omd d0
+// (exactly th at V = th, whatever the roundings), s2, m^2 and the branch (quadratic when
+// psi = s2/m^2 < 1.5).
 template <int PPT>
-__device__ __forceinline__ void qe_moments_fp32(const float (&V)[PPT], const SegF& c, float (&m)[PPT],
-                                                float (&s2)[PPT], float (&m2)[PPT], bool (&quad)[PPT]) {
+__device__ __forceinline__ void qe_moments_fp32(const float (&V)[PPT], const SegF& c, float (&d0)[PPT],
+                                                float (&m)[PPT], float (&s2)[PPT], float (&m2)[PPT],
+                                                bool (&quad)[PPT]) {
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        m[j] = __builtin_fmaf(c.omd, c.th - V[j], V[j]);
+        d0[j] = c.th - V[j];
+        m[j] = __builtin_fmaf(c.omd, d0[j], V[j]);
         s2[j] = __builtin_fmaf(V[j], c.c1, c.c2);
         m2[j] = m[j] * m[j];
         quad[j] = s2[j] < 1.5f * m2[j];
@@ -317,27 +321,29 @@ template <int PPT, bool MART, bool GEO>
 __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT], float (&X)[PPT], float (&Xp)[PPT],
                                                float (&P)[PPT], float (&Yp)[PPT], float (&PY)[PPT],
                                                const uint32_t (&wv)[PPT], const float (&zv)[PPT],
-                                               const float (&zx)[PPT], const float (&m)[PPT], const float (&s2)[PPT],
-                                               const float (&m2)[PPT], const bool (&quad)[PPT], const SegF& c) {
+                                               const float (&zx)[PPT], const float (&d0)[PPT], const float (&m)[PPT],
+                                               const float (&s2)[PPT], const float (&m2)[PPT], const bool (&quad)[PPT],
+                                               const SegF& c) {
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        float Vn;
+        float dm;  // V' - m
         // MART: A m - ln E[exp(A V')] = fs f(ft) + corr with f(t) = t - ln(1 + t), in natural
         // log units; mok: the expectation exists
         float ft = 0.0f, fs = 0.0f, corr = 0.0f;
         bool mok = false;
         if (quad[j]) {
             const float inv = __builtin_amdgcn_rcpf(s2[j]);
-            const float tt = (m2[j] + m2[j]) * inv;  // 2 / psi
+            const float tt = (m2[j] + m2[j]) * inv;  // 2 / psi (finite by the FP32 validation bound)
             const float tm1 = tt - 1.0f;
             const float b2 = tm1 + __builtin_amdgcn_sqrtf(tt * tm1);
             const float b = __builtin_amdgcn_sqrtf(b2);
-            // a = m / (1 + b^2) from the rounded b and a refined reciprocal, so that the scheme's
-            // mean E[V'] = a (b^2 + 1) = m holds up to unbiased rounding: a biased mean moves the
-            // variance's mean-reversion level by (relative bias) / (kappa dt)
-            const float aq = m[j] * rcp_refined(__builtin_fmaf(b, b, 1.0f));
-            const float bz = b + zv[j];
-            Vn = aq * bz * bz;
+            // V' = a (b + Z)^2 with a = m / (1 + b^2), taken as V' - m = a (Z (Z + 2 b) - 1): its
+            // mean is 0 whatever the rounding of a and b, so the scheme's E[V'] = m holds exactly
+            // (a biased mean would move the variance's mean-reversion level by (bias) / (kappa dt))
+            const float aq = m[j] * __builtin_amdgcn_rcpf(__builtin_fmaf(b, b, 1.0f));
+            // near b + Z = 0, m + (V' - m) can round a few ulp of m below 0 where a (b + Z)^2 is
+            // >= 0: harmless to the moments, and the diffusion below takes |k3 (V + V')|
+            dm = aq * __builtin_fmaf(zv[j], __builtin_fmaf(2.0f, b, zv[j]), -1.0f);
             if constexpr (MART) {
                 // x = 2 A a < 1: A m - ln E[exp(A V')] = -(f(-x) + x^2 b^2 / (1 - x)) / 2
                 const float x = (c.mA + c.mA) * aq;
@@ -355,7 +361,7 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
             const float lg = __builtin_amdgcn_logf(tm2 * rcp_refined(lhs));
             const float ib = sm * rcp_refined(m[j]);  // 2 / beta
             const float v = ib * (0.5f * 0.693147180559945309f) * lg;
-            Vn = lhs < tm2 ? v : 0.0f;
+            dm = (lhs < tm2 ? v : 0.0f) - m[j];
             if constexpr (MART) {
                 // cA = A / beta < 1, r1 = (1-p) / (1 - cA) = 2 m^2 / (sm (1 - cA)), y = r1 cA:
                 // A m - ln E[exp(A V')] = f(y) - r1 cA^2
@@ -367,9 +373,15 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
                 corr = -ft * cA;
             }
         }
+        // The reference's log-price step multiplies V' - V by rho/sigma and V + V' - 2 th by
+        // kappa rho dt / (2 sigma). As differences of FP32 variances they would carry ulp(V) errors
+        // amplified by 1/sigma (prices tens of percent off at sigma = 1e-6, or at sigma = 1e-4
+        // with dt = 2e-7). Formed from V' - m and th - V, each term has its own size:
+        // V' - V = (V' - m) + omd (th - V), V + V' - 2 th = (V' - m) - (1 + e^{-kappa dt}) (th - V).
+        const float Vn = m[j] + dm;
+        const float du = __builtin_fmaf(c.omd, d0[j], dm);
         const float sum = V[j] + Vn;
-        const float dif = Vn - V[j];
-        float inc = __builtin_fmaf(c.kd, dif, __builtin_fmaf(c.kc, sum - (c.th + c.th), c.g0));
+        float inc = __builtin_fmaf(c.kd, du, __builtin_fmaf(c.kc, __builtin_fmaf(-c.ope, d0[j], dm), c.g0));
         if constexpr (MART) {
             // f(t) = t - ln(1 + t): its series through t^7 for |t| < 1/16 (relative error < 2e-8),
             // else from the rounded 1 + t. The direct form alone would return f = t wherever 1 + t
@@ -383,10 +395,10 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
             const float fdir = __builtin_fmaf(-0.693147180559945309f, __builtin_amdgcn_logf(1.0f + ft), ft);
             const float f = __builtin_fabsf(ft) < 0.0625f ? t2 * h : fdir;
             const float cl = __builtin_fmaf(fs, f, corr) * 1.44269504088896341f;
-            const float im = __builtin_fmaf(c.ma, Vn - m[j], __builtin_fmaf(-c.mh, sum, c.mr)) + cl;
+            const float im = __builtin_fmaf(c.ma, dm, __builtin_fmaf(-c.mh, sum, c.mr)) + cl;
             inc = mok ? im : inc;
         }
-        const float sq = __builtin_amdgcn_sqrtf(c.k3 * sum);
+        const float sq = __builtin_amdgcn_sqrtf(__builtin_fabsf(c.k3 * sum));
         if constexpr (GEO) Yp[j] = Y[j];
         Y[j] = __builtin_fmaf(sq, zx[j], Y[j] + inc);
         Xp[j] = X[j];
@@ -432,13 +444,13 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
                                              const uint32_t (&wv)[PPT], const uint32_t (&wx)[PPT],
                                              const SegF& c, uint32_t* q, int lane, const bool (&valid)[PPT],
                                              StepCount& cnt) {
-    float m[PPT], s2[PPT], m2[PPT], zv[PPT], zx[PPT];
+    float d0[PPT], m[PPT], s2[PPT], m2[PPT], zv[PPT], zx[PPT];
     bool quad[PPT];
     uint32_t w[2 * PPT];
     bool t[2 * PPT];
     uint64_t mk[2 * PPT];
     float z[2 * PPT];
-    qe_moments_fp32<PPT>(V, c, m, s2, m2, quad);
+    qe_moments_fp32<PPT>(V, c, d0, m, s2, m2, quad);
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         bool cv, cx;
@@ -451,7 +463,7 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
         zv[j] = z[2 * j];
         zx[j] = z[2 * j + 1];
     }
-    qe_update_fp32<PPT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, wv, zv, zx, m, s2, m2, quad, c);
+    qe_update_fp32<PPT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, wv, zv, zx, d0, m, s2, m2, quad, c);
 }
 
 template <int PPT>
@@ -501,7 +513,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
     const uint32_t n_segs = R.n_segs;
     for (uint32_t s = 0; s < n_segs; ++s) {
         const DevSeg& S = a.segs[R.seg_begin + s];
-        const SegF c{S.omd, S.th, S.c1, S.c2, S.g0, S.kc, S.kd, S.k3, S.mA, S.ma, S.mh, S.mr};
+        const SegF c{S.omd, S.th, S.c1, S.c2, S.g0, S.kc, S.kd, S.k3, S.mA, S.ma, S.mh, S.mr, S.ope};
         const uint32_t n = S.n_steps;
         uint32_t i = 0;
         if ((k & 1u) && n > 0) {  // second half of a block opened by the previous run

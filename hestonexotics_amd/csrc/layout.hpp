@@ -43,7 +43,7 @@ struct DevSeg {
     float dnext_f;  // weight of the accumulation after the run's last step (new dt), 0 if none
     uint32_t n_steps;
     uint32_t fin_begin, fin_count;
-    uint32_t pad0;
+    float ope;  // 1 + exp(-kappa delta)
     double delta;  // FP64 kernel: step width
     double dnext;  // FP64 kernel: dt after the finalisations
 };

@@ -59,6 +59,8 @@ MUTATIONS = {
     "path_offset_unaligned": lambda r: setattr(r, "path_offset", 100),
     # the control variate needs an Asian average with weight sum >= 1/4 (one step has 0)
     "control_variate_one_step": lambda r: (setattr(r, "payoffs", H.ASIAN_GEO_CV), setattr(r, "steps", 1)),
+    # FP32 below sigma^2 dt = 1e-18 max(v0, theta): the variance step's moment ratio would overflow
+    "fp32_sigma_too_small": lambda r: (setattr(r, "sigma", 1e-9), setattr(r, "precision", H.FP32)),
     # the regression coefficient needs the Asian and geometric blocks beside the control variate
     "cv_regression_without_geo": lambda r: (setattr(r, "payoffs", H.ASIAN | H.ASIAN_GEO_CV),
                                             setattr(r, "flags", H.CV_REGRESSION)),
@@ -74,6 +76,11 @@ def test_validation_accepts_good_request():
     assert H.validate(r) == 0
     r.payoffs = H.ASIAN | H.EUROPEAN | H.ASIAN_GEO_CV | H.GEO_ASIAN
     r.flags = H.CV_REGRESSION | H.MARTINGALE | H.DISCOUNT | H.DRIFT | H.TRAPEZOID
+    assert H.validate(r) == 0
+    # tiny sigma: FP64 takes it; FP32 down to sigma^2 dt = 1e-18 max(v0, theta)
+    r.sigma, r.precision = 1e-9, H.FP64
+    assert H.validate(r) == 0
+    r.sigma, r.precision = 1e-8, H.FP32
     assert H.validate(r) == 0
 
 
