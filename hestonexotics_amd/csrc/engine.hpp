@@ -38,7 +38,7 @@ bool build_schedule(const double* T, uint32_t n_exp, const uint32_t* n_strikes, 
                     std::vector<HostSeg>& segs, uint64_t& total_steps);
 
 // ---- geometric-average control variate (geo_host.cpp) -------------------------------------
-constexpr uint32_t kGeoMaxPanels = 4096;   // 64-node panels per exact price before it reads NaN
+constexpr uint32_t kGeoMaxPanels = 20000;  // 64-node panels per exact price before it reads NaN
 constexpr uint32_t kGeoMaxPoints = 65536;  // grid steps of one control-variate expiry
 constexpr uint64_t kGeoMaxRequestPoints = 1u << 22;  // grid points over a request's control-variate expiries
 constexpr uint32_t kGeoMaxP0 = 128;        // panels per item evaluated in parallel

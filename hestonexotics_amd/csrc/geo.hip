@@ -92,25 +92,37 @@ __device__ __forceinline__ uint32_t find_item(const uint32_t* end, uint32_t n, u
 // phi(1/2 - i u_q) weighted by w_q (h/2) / (u_q^2 + 1/4); the 64 values go through LDS (sh: this
 // wave's 3 x 64 doubles) and lane i returns strike i's panel sum, node by node, of
 // Re(e^{i u_q k_i} weighted phi). *mag is the wave's max |phi| over the panel.
+// Panel 0 holds the integrand's 1/(u^2 + 1/4) peak at u = 0, which 64 nodes integrate to 1e-13
+// over [0, 16] but only to 1e-4 over [0, 120]: it is summed as n_sub = min(64, ceil(h / 16))
+// sub-panels of 64 nodes (wave-uniform, in order). The other panels keep width h.
+__device__ __forceinline__ uint32_t first_panel_parts(double h) {
+    return (uint32_t)fmin(64.0, ceil(h / 16.0));
+}
+
 __device__ double panel_sum(const GeoArgs& a, const DevGeo& G, const DevReq& R, const double* pts, uint32_t p, int lane,
                             double k, bool has_k, double (*sh)[64], double* mag) {
-    const double u = ((double)p + 0.5 * (a.gl[lane] + 1.0)) * G.h;
-    const Cplx ph = geo_cf(R, pts, G.n_pts, cx(0.5, -u));
-    const double f = 0.5 * G.h * a.gl[64 + lane] / (u * u + 0.25);
-    sh[0][lane] = f * ph.re;
-    sh[1][lane] = f * ph.im;
-    sh[2][lane] = u;
-    *mag = wave_max(cabs_(ph));
-    wave_sync_lds();
-    double sum = 0.0;
-    if (has_k) {
-        for (int q = 0; q < 64; ++q) {
-            double sn, cs;
-            sincos(sh[2][q] * k, &sn, &cs);
-            sum += cs * sh[0][q] - sn * sh[1][q];
+    const uint32_t n_sub = p == 0 ? first_panel_parts(G.h) : 1u;
+    const double hs = G.h / (double)n_sub;
+    double sum = 0.0, mg = 0.0;
+    for (uint32_t j = 0; j < n_sub; ++j) {
+        const double u = ((double)(p + j) + 0.5 * (a.gl[lane] + 1.0)) * hs;  // p = 0 or j = 0
+        const Cplx ph = geo_cf(R, pts, G.n_pts, cx(0.5, -u));
+        const double f = 0.5 * hs * a.gl[64 + lane] / (u * u + 0.25);
+        sh[0][lane] = f * ph.re;
+        sh[1][lane] = f * ph.im;
+        sh[2][lane] = u;
+        mg = fmax(mg, wave_max(cabs_(ph)));
+        wave_sync_lds();
+        if (has_k) {
+            for (int q = 0; q < 64; ++q) {
+                double sn, cs;
+                sincos(sh[2][q] * k, &sn, &cs);
+                sum += cs * sh[0][q] - sn * sh[1][q];
+            }
         }
+        wave_sync_lds();  // sh is rewritten by the next sub-panel or panel
     }
-    wave_sync_lds();  // sh is rewritten by the next panel
+    *mag = mg;
     return sum;
 }
 

@@ -235,3 +235,29 @@ def test_control_variate_rejects_one_step(ctx):
         ctx.price(req)
     with pytest.raises(H.HexoError):
         ctx.geo_asian_price(req)
+
+
+@pytest.mark.parametrize("case", [
+    dict(v0=0.0, theta=1.21e-4, rho=-0.622, kappa=0.367, sigma=0.0823, T=0.0543, steps=60),
+    dict(v0=2.2e-4, theta=2.96e-3, rho=0.902, kappa=0.3, sigma=1.8, T=0.0112, steps=330),
+    dict(v0=1e-4, theta=1e-4, rho=-0.99, kappa=1.0, sigma=0.05, T=0.02, steps=20),
+])
+def test_exact_price_near_deterministic(ctx, oracle, case):
+    """Nearly deterministic geometric averages (variance near 0, short expiries): wide panels whose
+    first one is summed in sub-panels. The GPU's exact prices equal the oracle's to 1e-9 S0 and
+    are call prices (non-negative, at least the intrinsic value E[G] - K, decreasing in K)."""
+    say("gpu geo exact near-deterministic", case)
+    p = [case["v0"], case["theta"], case["rho"], case["kappa"], case["sigma"]]
+    t, w = oracle.asian_weights([case["T"]], [1], case["steps"], 0, H.DRIFT)
+    W = w.sum()
+    EG = W * 100.0 * oracle.geo_cf(p, 0.02, t, w / W, 1.0 + 0j).real
+    K = list(np.linspace(0.9, 1.1, 11) * EG)
+    req = H.Request(v0=case["v0"], theta=case["theta"], rho=case["rho"], kappa=case["kappa"], sigma=case["sigma"],
+                    S0=100.0, expiries=[case["T"]], strikes=[K], steps=case["steps"], n_paths=64, r=0.02,
+                    flags=H.DRIFT, payoffs=H.ASIAN_GEO_CV, precision=H.FP64)
+    g = ctx.geo_asian_price(req)
+    o = oracle.geo_price_weights(p, 100.0, 0.02, t, w, K)
+    say(f"  max |gpu - oracle| / S0 {np.abs(g - o).max() / 100.0:.2e}, min price {g.min():.3e}")
+    np.testing.assert_allclose(g, o, rtol=0, atol=1e-9 * 100.0)
+    tol = 1e-9 * EG
+    assert np.all(g >= -tol) and np.all(g >= EG - np.array(K) - tol) and np.all(np.diff(g) <= tol)

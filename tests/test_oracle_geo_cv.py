@@ -150,3 +150,33 @@ def test_regression_coefficient_estimator_is_never_worse():
         assert np.all(np.abs(price - p[:3]) < 4 * s[:3])
         if T == 0.1:
             assert beta[1] < 0.9, beta
+
+
+NEAR_DETERMINISTIC = [
+    # variance near 0 and short expiries: L = sum_j v_j ln(X_j/S0) is nearly deterministic, its
+    # characteristic function decays slowly, and the panels are wide
+    dict(v0=0.0, theta=1.21e-4, rho=-0.622, kappa=0.367, sigma=0.0823, T=0.0543, steps=60),
+    dict(v0=2.2e-4, theta=2.96e-3, rho=0.902, kappa=0.3, sigma=1.8, T=0.0112, steps=330),
+    dict(v0=1e-4, theta=1e-4, rho=-0.99, kappa=1.0, sigma=0.05, T=0.02, steps=20),
+]
+
+
+@pytest.mark.parametrize("case", NEAR_DETERMINISTIC)
+def test_exact_geometric_price_is_an_arbitrage_free_call(case):
+    """The Lewis integrand has a 1/(u^2 + 1/4) peak at u = 0; when L is nearly deterministic the
+    panels are wide (width up to 1/sd) and a single 64-node first panel integrated that peak to
+    1e-4 only: prices came out negative (-2e-3). Over a strike grid around the forward, the exact
+    price must be a call price: 0 <= C <= E[G], C >= E[G] - K, decreasing and convex in K."""
+    from oracle import oracle as O
+    p = [case["v0"], case["theta"], case["rho"], case["kappa"], case["sigma"]]
+    t, w = O.asian_weights([case["T"]], [1], case["steps"], 0, 1)
+    W = w.sum()
+    EG = W * 100.0 * O.geo_cf(p, 0.02, t, w / W, 1.0 + 0j).real
+    K = np.linspace(0.9, 1.1, 11) * EG
+    C = O.geo_price_weights(p, 100.0, 0.02, t, w, K)
+    tol = 1e-9 * EG
+    assert np.all(np.isfinite(C))
+    assert np.all(C >= -tol) and np.all(C <= EG + tol)
+    assert np.all(C >= EG - K - tol)
+    assert np.all(np.diff(C) <= tol)
+    assert np.all(np.diff(C, 2) >= -tol)
