@@ -71,7 +71,9 @@ typedef enum hexo_status {
  * variances of A, G and A - G); formed in the final reduction on the GPU. */
 #define HEXO_CV_REGRESSION 16u
 
-/* precision of the path kernel */
+/* precision of the path kernel. The FP32 modes need sigma^2 dt >= 1e-18 max(v0, theta) for every
+ * step width dt (below it the variance step's moment ratio would overflow single precision;
+ * such requests are refused with HEXO_ERR_INVALID and price in HEXO_FP64). */
 #define HEXO_FP64 0u             /* plain FP64 kernel: reference form, FP64 everywhere */
 #define HEXO_FP32 1u             /* FP32 stepping, FP64 cross-path reduction and estimator */
 #define HEXO_FP32_COMPENSATED 2u /* FP32 stepping, compensated-FP32 reduction, FP64 estimator */
