@@ -11,7 +11,8 @@
 // The call price is Lewis' inversion
 //   E[(c e^L - K)^+] = c (phi(1) - e^{k/2}/pi int_0^inf Re(e^{iuk} phi(1/2 - iu)) / (u^2 + 1/4) du),
 // k = ln(K / c), by 64-node Gauss-Legendre panels of width h (host-chosen from the scale of L and
-// the strikes) until the characteristic function has decayed below 1e-15 on a whole panel.
+// the strikes) until the characteristic function has decayed below 1e-15 u0 on a whole panel
+// starting at u0 (the integral's remaining tail is then below 1e-15).
 //
 // Work items are (request, expiry, block of <= 64 strikes), in up to three rounds. Pass 1 of a
 // round evaluates the next p0 panels of every unconverged item at once, one wavefront per panel
@@ -172,7 +173,8 @@ __global__ __launch_bounds__(256) void k_geo_panels(GeoArgs a, uint32_t round) {
 }
 
 // Pass 2 of round r: one wavefront per item; lane i adds strike i's panel sums in panel order up
-// to the first panel (after the first) whose |phi| stays below 1e-15. An unconverged item keeps
+// to the first panel (after the first) whose |phi| stays below 1e-15 u0, u0 its start: beyond it the
+// integrand |phi| / (u^2 + 1/4) leaves at most 1e-15 to the integral. An unconverged item keeps
 // its sums for the next round; in the last round the wavefront evaluates further panels itself.
 // The price is the same ordered sum whatever p0 and the number of rounds are.
 __global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a, uint32_t round) {
@@ -195,7 +197,7 @@ __global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a, uint32_t round) {
     uint32_t p = base;
     for (uint32_t j = 0; j < G.p0 && p < G.max_panels && !converged; ++j, ++p) {
         integ += a.panel[(size_t)(G.wave_begin + j) * 64 + (uint32_t)lane];
-        converged = p > 0 && a.panel_max[G.wave_begin + j] < 1e-15;
+        converged = p > 0 && a.panel_max[G.wave_begin + j] < 1e-15 * (double)p * G.h;
     }
     const bool last = round + 1 == kGeoRounds;
     if (!converged && !last && p < G.max_panels) {
@@ -210,7 +212,7 @@ __global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a, uint32_t round) {
     for (; p < G.max_panels && !converged; ++p) {
         double mag;
         integ += panel_sum(a, G, R, pts, p, lane, k, has_k, sh[w], &mag);
-        converged = p > 0 && mag < 1e-15;  // wave-uniform
+        converged = p > 0 && mag < 1e-15 * (double)p * G.h;  // wave-uniform
     }
     if (lane == 0) st[66] = 1.0;  // later rounds skip the item
     if ((uint32_t)lane >= G.n_strikes) return;

@@ -221,7 +221,8 @@ int orc_geo_price_weights(const double p[5], double S0, double r, const double* 
                 for (uint32_t i = 0; i < n_strikes; ++i)
                     if (isfinite(kk[i])) integ[i] += wq * creal(cexp(I * u * kk[i]) * ph) / (u * u + 0.25);
             }
-        if (pnl > 0 && amax < 1e-15) converged = 1;
+        /* beyond u0 = pnl h the integrand |phi|/(u^2 + 1/4) leaves at most sup|phi| / u0 */
+        if (pnl > 0 && amax < 1e-15 * pnl * h) converged = 1;
     }
     for (uint32_t i = 0; i < n_strikes; ++i)
         out[i] = !converged ? NAN
