@@ -450,7 +450,9 @@ class Context:
         info = hexo_ticket_info()
         rc = self.lib.hexo_wait(self.h, ticket, timeout_s, _dp(p), _dp(s), C.byref(info))
         if rc != 0:
-            raise HexoError(rc, "wait")
+            # refusals of the call itself carry a message; a ticket's own failure status does not
+            msg = self.lib.hexo_ctx_last_error(self.h).decode() if rc in (-1, -9) else "wait"
+            raise HexoError(rc, msg)
         return p, s, {k: getattr(info, k) for k, _ in info._fields_}
 
     def cancel(self, ticket: int):

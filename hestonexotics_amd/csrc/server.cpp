@@ -369,6 +369,7 @@ int hexo_poll(hexo_ctx* ctx, uint64_t ticket, int* done) {
 int hexo_wait(hexo_ctx* ctx, uint64_t ticket, double timeout_s, double* prices, double* std_errs,
               hexo_ticket_info* info) {
     if (!ctx) return HEXO_ERR_INVALID;
+    if (std::isnan(timeout_s)) return ctx->set_error(HEXO_ERR_INVALID, "timeout_s is NaN");
     const std::shared_ptr<hexo_server_state> s = ctx->server_ref();
     if (!s) return ctx->set_error(HEXO_ERR_STATE, "server not running");
     std::unique_lock<std::mutex> lk(s->mu);
@@ -376,7 +377,8 @@ int hexo_wait(hexo_ctx* ctx, uint64_t ticket, double timeout_s, double* prices, 
     if (it == s->tickets.end()) return HEXO_ERR_NOT_FOUND;
     hexo_server_state::Ticket* t = it->second.get();
     auto ready = [t] { return t->state >= 2; };
-    if (timeout_s < 0)
+    // beyond 1e7 s (about 116 days) the deadline would overflow the clock's range: wait forever
+    if (timeout_s < 0 || timeout_s > 1e7)
         s->cv.wait(lk, ready);
     else if (!s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), ready))
         return HEXO_ERR_TIMEOUT;

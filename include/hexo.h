@@ -245,7 +245,9 @@ typedef struct hexo_server_stats {
 int hexo_server_start(hexo_ctx* ctx, const hexo_server_config* cfg);
 int hexo_submit(hexo_ctx* ctx, const hexo_request* req, uint64_t* ticket);
 int hexo_poll(hexo_ctx* ctx, uint64_t ticket, int* done);
-/* Waits up to timeout_s (< 0: forever). On success copies outputs and releases the ticket. */
+/* Waits up to timeout_s seconds (< 0 or > 1e7: forever; NaN: HEXO_ERR_INVALID). Once the ticket
+ * has finished it returns the ticket's status, copies the outputs if that is HEXO_OK and releases
+ * the ticket (a later call gives HEXO_ERR_NOT_FOUND); on HEXO_ERR_TIMEOUT the ticket stays. */
 int hexo_wait(hexo_ctx* ctx, uint64_t ticket, double timeout_s, double* prices, double* std_errs,
               hexo_ticket_info* info);
 int hexo_cancel(hexo_ctx* ctx, uint64_t ticket);

@@ -70,12 +70,15 @@ def test_submit_wait_poll_cancel(ctx):
                 break
             except H.HexoError:
                 continue
-        for r, t in zip(reqs, tickets):
+        # a NaN timeout is refused and leaves the ticket; one beyond the clock's range waits forever
+        with pytest.raises(H.HexoError, match="NaN"):
+            ctx.wait(tickets[0], reqs[0].n_outputs, timeout_s=float("nan"))
+        for i, (r, t) in enumerate(zip(reqs, tickets)):
             if t == cancelled:
                 with pytest.raises(H.HexoError):
                     ctx.wait(t, r.n_outputs)
                 continue
-            p, s, info = ctx.wait(t, r.n_outputs, timeout_s=60.0)
+            p, s, info = ctx.wait(t, r.n_outputs, timeout_s=1e300 if i == 0 else 60.0)
             p1, _ = ctx.price(r)
             np.testing.assert_array_equal(p, p1)
         with pytest.raises(H.HexoError):
