@@ -233,6 +233,12 @@ int hexo_batch_sync(hexo_batch* hb) {
 
 int hexo_batch_results(hexo_batch* hb, double* prices, double* std_errs) {
     if (!hb || !hb->b || !prices) return HEXO_ERR_INVALID;
+    hexo_ctx* ctx = hb->b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    DeviceGuard dg(ctx->device);
+    // the outputs land in pinned host buffers at the end of each run: wait for a run in flight
+    if (hipStreamSynchronize(hb->b->stream) != hipSuccess)
+        return ctx->set_error(HEXO_ERR_HIP, "wait for the batch's run failed");
     hexo::batch_copy_results(*hb->b, prices, std_errs);
     return HEXO_OK;
 }

@@ -168,6 +168,7 @@ int hexo_batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hexo_
 int hexo_batch_run(hexo_batch* b);
 int hexo_batch_run_eager(hexo_batch* b); /* same launches without the graph (HIP-event timed) */
 int hexo_batch_sync(hexo_batch* b);
+/* Outputs of the last run (waits for a run in flight; layout as hexo_price's). */
 int hexo_batch_results(hexo_batch* b, double* prices, double* std_errs);
 int hexo_batch_stats_get(hexo_batch* b, hexo_batch_stats* st);
 /* Exact branch counts of the batch's paths (runs a counting variant of the kernels once):

@@ -265,6 +265,10 @@ def test_graph_replays_match_eager(ctx):
         b.sync()
         p2, _ = b.results()
         np.testing.assert_array_equal(p, p2)
+    # results() waits for a replay still in flight
+    b.run()
+    p3, _ = b.results()
+    np.testing.assert_array_equal(p, p3)
     b.close()
 
 
