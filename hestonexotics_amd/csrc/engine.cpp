@@ -94,7 +94,9 @@ std::string validate(const hexo_request* r) {
     if (!fin(r->S0) || !(r->S0 > 0.0)) return "S0 must be finite and > 0";
     if (!fin(r->r)) return "r must be finite";
     if (r->n_paths == 0) return "n_paths must be >= 1";
-    if (r->n_paths > (1ull << 38)) return "n_paths must be <= 2^38";
+    // tile indices and work items are 32-bit and a batch holds fewer than 2^31 work items: at most
+    // 2^31 - 1 tiles of 64 paths
+    if (r->n_paths > kMaxPaths) return "n_paths must be <= 2^37 - 64 (at most 2^31 - 1 tiles of 64 paths)";
     if (r->path_offset > ~0ull - r->n_paths) return "path_offset + n_paths overflows";
     if (r->path_offset % kTile) return "path_offset must be a multiple of 64 (one tile)";
     if (r->steps == 0 || r->steps > (1u << 22)) return "steps must lie in [1, 2^22]";
@@ -132,6 +134,11 @@ std::string validate(const hexo_request* r) {
         if (r->sigma * r->sigma * dmin < 1e-18 * std::max(r->v0, r->theta))
             return "sigma^2 dt below 1e-18 max(v0, theta) overflows the FP32 variance step; use HEXO_FP64";
     }
+    // At rho = +-1 the geometric average's log is a functional of the variance path alone: its
+    // characteristic function does not decay and the exact price's Fourier integral never
+    // converges (it would read NaN)
+    if ((r->payoffs & HEXO_ASIAN_GEO_CV) && std::fabs(r->rho) >= 1.0)
+        return "the geometric control variate's exact price does not converge at rho = +-1";
     if (r->payoffs & (HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN)) {
         GeoPlan g;
         std::string why;
@@ -661,12 +668,16 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
             return HEXO_ERR_HIP;
         }
     if (hipMemcpyAsync(b->d_mem, b->h_stage, b->upload_bytes, hipMemcpyHostToDevice, stream) != hipSuccess) {
+        (void)hipStreamSynchronize(stream);  // a copy may still be writing the region or reading the staging buffer
         batch_destroy(*b);
         err = "upload of the batch tables failed";
         return HEXO_ERR_HIP;
     }
     // the exact geometric prices depend on the requests only: computed once, before any run
     if (ga.n_items && launch_geo(ga, stream) != hipSuccess) {
+        // the upload is in flight: the region and the (possibly shared) staging buffer are handed
+        // back only once it is done
+        (void)hipStreamSynchronize(stream);
         batch_destroy(*b);
         err = "launch of the geometric control-variate prices failed";
         return HEXO_ERR_HIP;

@@ -21,7 +21,8 @@
 // >= <# volume> (src/WebAPI.cpp:62-84,101-105). The output line per option has the reference's
 // columns (src/Main.cpp:89-93). Environment: HEXO_PATHS (1e5), HEXO_STEPS (1e3),
 // HEXO_PRECISION (fp64 | fp32, default fp64), HEXO_SEED (1); variance reduction, off by default:
-// HEXO_CONTROL_VARIATE (none | geo | regression), HEXO_MARTINGALE (0 | 1).
+// HEXO_CONTROL_VARIATE (none | geo = regression: the regression coefficient | geo-unit:
+// coefficient 1), HEXO_MARTINGALE (0 | 1).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -174,7 +175,7 @@ std::vector<ffloat> run_price(const UnderlyingData& d, unsigned paths, unsigned 
 }
 
 // The scheme from the environment: precision, Andersen's martingale correction, and the
-// geometric-average control variate (coefficient 1 or the regression coefficient).
+// geometric-average control variate (the regression coefficient, or coefficient 1 on request).
 template <typename acc, class Policy>
 std::vector<ffloat> run_scheme(bool mart, const UnderlyingData& d, unsigned paths, unsigned n_opts, unsigned steps) {
     return mart ? run_price<HSimulation::HQEAndersonMartingale<acc, Policy>>(d, paths, n_opts, steps)
@@ -184,9 +185,10 @@ std::vector<ffloat> run_scheme(bool mart, const UnderlyingData& d, unsigned path
 template <typename acc>
 std::vector<ffloat> run_policy(const std::string& cv, bool mart, const UnderlyingData& d, unsigned paths,
                                unsigned n_opts, unsigned steps) {
-    if (cv == "geo") return run_scheme<acc, AAsianCallGeometricCV>(mart, d, paths, n_opts, steps);
-    if (cv == "regression") return run_scheme<acc, AAsianCallGeometricCVRegression>(mart, d, paths, n_opts, steps);
-    if (!cv.empty() && cv != "none") throw std::runtime_error("HEXO_CONTROL_VARIATE must be none, geo or regression");
+    if (cv == "geo" || cv == "regression") return run_scheme<acc, AAsianCallGeometricCV>(mart, d, paths, n_opts, steps);
+    if (cv == "geo-unit") return run_scheme<acc, AAsianCallGeometricCVUnitCoefficient>(mart, d, paths, n_opts, steps);
+    if (!cv.empty() && cv != "none")
+        throw std::runtime_error("HEXO_CONTROL_VARIATE must be none, geo (= regression) or geo-unit");
     return run_scheme<acc, AAsianCallNonAdaptive>(mart, d, paths, n_opts, steps);
 }
 

@@ -17,6 +17,7 @@ namespace hexo {
 constexpr int kTile = 64;         // paths per tile = one wavefront
 constexpr int kGroupTiles = 64;   // tiles per first-level reduction group
 constexpr int kMaxPPT = 4;        // paths per thread (FP32 kernel)
+constexpr unsigned long long kMaxPaths = (1ull << 37) - kTile;  // per request: 2^31 - 1 tiles
 
 // One run of steps with a constant step width (the reference's dt = T_c / steps between two
 // expiry finalisations). FP32 constants are computed on the host in FP64 (per request, per run):

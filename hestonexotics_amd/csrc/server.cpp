@@ -61,7 +61,9 @@ struct hexo_server_state {
     std::condition_variable cv;
     bool stopping = false;
     uint64_t next_ticket = 1;
-    std::unordered_map<uint64_t, std::unique_ptr<Ticket>> tickets;
+    // shared: a waiter keeps its ticket alive across the condition wait, while another waiter on
+    // the same ticket may consume and erase it
+    std::unordered_map<uint64_t, std::shared_ptr<Ticket>> tickets;
     std::deque<uint64_t> queue;
     std::vector<Flight> flights;
     size_t budget = 0, reserved = 0, peak = 0;
@@ -325,7 +327,7 @@ int hexo_submit(hexo_ctx* ctx, const hexo_request* req, uint64_t* ticket) {
         return ctx->set_error(HEXO_ERR_INVALID, why);
     }
     if (s->stopping) return ctx->set_error(HEXO_ERR_STATE, "server stopping");
-    auto t = std::make_unique<hexo_server_state::Ticket>();
+    auto t = std::make_shared<hexo_server_state::Ticket>();
     t->req = *req;
     const uint64_t n_opts = hexo::n_opts_of(req);
     t->T.assign(req->expiries, req->expiries + req->n_expiries);
@@ -375,20 +377,29 @@ int hexo_wait(hexo_ctx* ctx, uint64_t ticket, double timeout_s, double* prices, 
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->tickets.find(ticket);
     if (it == s->tickets.end()) return HEXO_ERR_NOT_FOUND;
-    hexo_server_state::Ticket* t = it->second.get();
-    auto ready = [t] { return t->state >= 2; };
+    // Several threads may wait on one ticket: the first to wake up with the result consumes it.
+    // The others find it gone from the map on their next wake-up and report HEXO_ERR_NOT_FOUND;
+    // their reference keeps the ticket's memory valid until then.
+    const std::shared_ptr<hexo_server_state::Ticket> t = it->second;
+    auto gone = [&] {
+        auto j = s->tickets.find(ticket);
+        return j == s->tickets.end() || j->second != t;
+    };
+    auto ready = [&] { return t->state >= 2 || gone(); };
     // beyond 1e7 s (about 116 days) the deadline would overflow the clock's range: wait forever
     if (timeout_s < 0 || timeout_s > 1e7)
         s->cv.wait(lk, ready);
     else if (!s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), ready))
         return HEXO_ERR_TIMEOUT;
+    if (gone()) return HEXO_ERR_NOT_FOUND;
     const int status = t->state == 2 ? HEXO_OK : t->info.status;
     if (info) *info = t->info;
     if (status == HEXO_OK) {
         if (prices) std::memcpy(prices, t->prices.data(), sizeof(double) * t->n_out);
         if (std_errs) std::memcpy(std_errs, t->ses.data(), sizeof(double) * t->n_out);
     }
-    s->tickets.erase(it);
+    s->tickets.erase(ticket);
+    s->cv.notify_all();  // other waiters on this ticket see it gone
     return status;
 }
 

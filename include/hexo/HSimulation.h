@@ -16,9 +16,17 @@
  * Variance reduction, which the reference suggests beside price (src/inc/HSimulation.h:48-60),
  * as further policies and a scheme (extensions, not in the reference):
  *   OptionPolicy AAsianCallGeometricCV           -> Asian calls with the geometric-average control
- *                                                   variate, coefficient 1 (HEXO_ASIAN_GEO_CV)
- *   OptionPolicy AAsianCallGeometricCVRegression -> the same with the regression coefficient
- *                                                   (HEXO_CV_REGRESSION)
+ *                                                   variate and the regression coefficient
+ *                                                   beta = Cov(A, G) / Var G from the same paths
+ *                                                   (HEXO_ASIAN_GEO_CV with HEXO_CV_REGRESSION):
+ *                                                   its standard error is never above the plain
+ *                                                   Asian one
+ *   OptionPolicy AAsianCallGeometricCVRegression -> the same (the explicit name)
+ *   OptionPolicy AAsianCallGeometricCVUnitCoefficient -> coefficient 1 instead (opt-in): cheaper by
+ *                                                   one payoff block, but worse than plain Monte
+ *                                                   Carlo where A and G decorrelate (few steps,
+ *                                                   the reference's final price increment
+ *                                                   dominating the average)
  *   Scheme HQEAndersonMartingale<accumulate_t, OptionPolicy> -> Andersen's martingale-corrected
  *                                                   step (HEXO_MARTINGALE)
  * The estimator is the reference's: undiscounted, no drift term, its time grid and Asian
@@ -55,15 +63,16 @@ struct EuropeanCallNonAdaptive {
     static constexpr uint32_t hexo_flags = 0;
     static constexpr uint32_t hexo_block = 0;
 };
-struct AAsianCallGeometricCV {
-    static constexpr uint32_t hexo_payoff = HEXO_ASIAN_GEO_CV;
-    static constexpr uint32_t hexo_flags = 0;
-    static constexpr uint32_t hexo_block = 0;
-};
-struct AAsianCallGeometricCVRegression {  // blocks: Asian, control variate, geometric
+struct AAsianCallGeometricCV {  // blocks: Asian, control variate, geometric
     static constexpr uint32_t hexo_payoff = HEXO_ASIAN | HEXO_ASIAN_GEO_CV | HEXO_GEO_ASIAN;
     static constexpr uint32_t hexo_flags = HEXO_CV_REGRESSION;
     static constexpr uint32_t hexo_block = 1;
+};
+struct AAsianCallGeometricCVRegression : AAsianCallGeometricCV {};
+struct AAsianCallGeometricCVUnitCoefficient {
+    static constexpr uint32_t hexo_payoff = HEXO_ASIAN_GEO_CV;
+    static constexpr uint32_t hexo_flags = 0;
+    static constexpr uint32_t hexo_block = 0;
 };
 
 namespace HSimulation {

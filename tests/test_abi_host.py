@@ -66,6 +66,13 @@ MUTATIONS = {
                                             setattr(r, "flags", H.CV_REGRESSION)),
     "cv_regression_without_cv": lambda r: (setattr(r, "payoffs", H.ASIAN | H.GEO_ASIAN),
                                            setattr(r, "flags", H.CV_REGRESSION)),
+    # the control variate's exact price does not converge at rho = +-1
+    "control_variate_rho_plus_one": lambda r: (setattr(r, "payoffs", H.ASIAN_GEO_CV), setattr(r, "rho", 1.0)),
+    "control_variate_rho_minus_one": lambda r: (setattr(r, "payoffs", H.ASIAN | H.ASIAN_GEO_CV | H.GEO_ASIAN),
+                                                setattr(r, "flags", H.CV_REGRESSION), setattr(r, "rho", -1.0)),
+    # tile indices are 32-bit and a batch holds fewer than 2^31 work items: 2^31 - 1 tiles at most
+    "paths_2_38": lambda r: setattr(r, "n_paths", 2 ** 38),
+    "paths_one_tile_past_limit": lambda r: setattr(r, "n_paths", 2 ** 37 - 63),
 }
 
 
@@ -81,6 +88,16 @@ def test_validation_accepts_good_request():
     r.sigma, r.precision = 1e-9, H.FP64
     assert H.validate(r) == 0
     r.sigma, r.precision = 1e-8, H.FP32
+    assert H.validate(r) == 0
+    # rho = +-1: plain Monte Carlo payoffs (the geometric one included) but no exact price
+    r = good()
+    r.payoffs, r.rho = H.ASIAN | H.EUROPEAN | H.GEO_ASIAN, 1.0
+    assert H.validate(r) == 0
+    r.payoffs, r.rho = H.ASIAN_GEO_CV, 0.999999
+    assert H.validate(r) == 0
+    # the largest path count: 2^31 - 1 tiles of 64 paths
+    r = good()
+    r.n_paths = 2 ** 37 - 64
     assert H.validate(r) == 0
 
 

@@ -40,11 +40,13 @@ def test_cli_prices_equal_c_abi(ctx):
     assert out.returncode == 0 and len([l for l in out.stdout.splitlines() if l.startswith("S:")]) == 3
 
 
-@pytest.mark.parametrize("cv,mart,prec", [("geo", "0", "fp64"), ("regression", "1", "fp32"), ("none", "1", "fp64")])
+@pytest.mark.parametrize("cv,mart,prec", [("geo", "0", "fp64"), ("regression", "1", "fp32"), ("geo-unit", "0", "fp64"),
+                                         ("none", "1", "fp64")])
 def test_cli_variance_reduction_equals_c_abi(ctx, cv, mart, prec):
     """The wrapper's variance-reduction policies and martingale-corrected scheme, selected in the CLI
     with HEXO_CONTROL_VARIATE and HEXO_MARTINGALE, give the C ABI's prices of the same request: the
-    control-variate block (coefficient 1 or the regression coefficient) or the corrected Asian."""
+    control-variate block (the regression coefficient for "geo" and "regression", coefficient 1 for
+    "geo-unit") or the corrected Asian."""
     print("gpu cli variance reduction", cv, mart, prec, flush=True)
     out = run_cli("-p", "asian", "all", "DEMO",
                   env_extra={"HEXO_PATHS": "20000", "HEXO_STEPS": "200", "HEXO_CONTROL_VARIATE": cv,
@@ -52,8 +54,9 @@ def test_cli_variance_reduction_equals_c_abi(ctx, cv, mart, prec):
     assert out.returncode == 0, out.stderr
     got = [float(re.search(r"asian-option-price: ([-0-9.]+)", l).group(1))
            for l in out.stdout.splitlines() if l.startswith("S:")]
-    payoffs, flags, block = {"geo": (H.ASIAN_GEO_CV, 0, 0),
+    payoffs, flags, block = {"geo": (H.ASIAN | H.ASIAN_GEO_CV | H.GEO_ASIAN, H.CV_REGRESSION, 1),
                              "regression": (H.ASIAN | H.ASIAN_GEO_CV | H.GEO_ASIAN, H.CV_REGRESSION, 1),
+                             "geo-unit": (H.ASIAN_GEO_CV, 0, 0),
                              "none": (H.ASIAN, 0, 0)}[cv]
     req = H.Request(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0, expiries=[65 / 261, 1.0],
                     strikes=[[90, 100, 110], [100, 120]], steps=200, n_paths=20000, seed=1, payoffs=payoffs,

@@ -3,8 +3,12 @@
 Pins (none of them restates the oracle's formulas):
  * QE variance step: the exact CIR conditional mean and variance (Andersen 2008, eqs. 17-18) are
    reproduced by both branches; the exponential branch's mass at 0 is p = (psi-1)/(psi+1).
- * Log-price step: with rho = 0 the step is conditionally lognormal with E[X'/X] = 1 exactly; in
-   the sigma -> 0 limit with v0 = theta the model is GBM, so European prices equal Black-Scholes.
+ * Log-price step: exactly, given V and V', the increment is Andersen's eq. (33) with the central
+   weights gamma1 = gamma2 = 1/2 (intercept and slope of the affine map Z -> d ln X over a grid of
+   the draw's words, normals from scipy); planted CPU copies with other weights, a flipped rho/eps
+   sign or no K0 fail the same check. Also: with rho = 0 the step is conditionally lognormal with
+   E[X'/X] = 1 exactly; in the sigma -> 0 limit with v0 = theta the model is GBM, so European
+   prices equal Black-Scholes.
  * Time grid (src/HSimulation.tpp:35-44) and the Asian finalisation quirk
    (src/inc/AsianContract.h:29-34): the case table of n* per (T, N), and the K = 0 invariant
    E[A] = S0 (N-1)/N when the last step lands on T, S0 otherwise.
@@ -52,6 +56,102 @@ def test_qe_step_reproduces_cir_moments(V, delta, kappa, theta, eps):
         p = (psi - 1) / (psi + 1)
         assert r["quad_frac"] == 0.0
         assert abs(r["zero_frac"] - p) < 5 * np.sqrt(p * (1 - p) / n)
+
+
+# ---- the log-price step, exactly -------------------------------------------------------------
+# Andersen (2008), "Simple and efficient simulation of the Heston stochastic volatility model",
+# J. Comput. Finance 11(3), eq. (33): given V(t) and V(t + D),
+#   ln X(t + D) = ln X(t) + K0 + K1 V(t) + K2 V(t + D) + sqrt(K3 V(t) + K4 V(t + D)) Z,
+#   K0 = -rho kappa theta D / eps,
+#   K1 = g1 D (kappa rho / eps - 1/2) - rho / eps,   K2 = g2 D (kappa rho / eps - 1/2) + rho / eps,
+#   K3 = g1 D (1 - rho^2),                           K4 = g2 D (1 - rho^2),
+# with the central discretisation g1 = g2 = 1/2, the one src/HSimulation.tpp:75-80 takes.
+GAMMA1 = GAMMA2 = 0.5
+
+
+def andersen_eq33(params, delta, gamma1=GAMMA1, gamma2=GAMMA2):
+    _, theta, rho, kappa, eps = params
+    k0 = -rho * kappa * theta * delta / eps
+    k1 = gamma1 * delta * (kappa * rho / eps - 0.5) - rho / eps
+    k2 = gamma2 * delta * (kappa * rho / eps - 0.5) + rho / eps
+    k3 = gamma1 * delta * (1.0 - rho * rho)
+    k4 = gamma2 * delta * (1.0 - rho * rho)
+    return k0, k1, k2, k3, k4
+
+
+def words_midpoint(log2_n):
+    """The midpoints of 2^log2_n equal intervals of the 32-bit words, and their uniforms
+    u = (w + 1/2) 2^-32 as the oracle maps them; the normals come from scipy's ndtri."""
+    from scipy.special import ndtri
+    s = 32 - log2_n
+    w = (np.arange(2 ** log2_n, dtype=np.uint64) << np.uint64(s)) | np.uint64(1 << (s - 1))
+    return w.astype(np.uint32), ndtri((w.astype(np.float64) + 0.5) * 2.0 ** -32)
+
+
+def check_log_step(step, params, delta, V, w_v, log2_n=12):
+    """Fixes V and the variance draw's word (so V' is fixed) and runs `step` over a midpoint grid
+    of the log-price draw's word. The log-price increment must be the affine function
+    a + b Z of the draw's normal Z with a = K0 + K1 V + K2 V' and b^2 = K3 V + K4 V' of eq. (33)."""
+    words, z = words_midpoint(log2_n)
+    out = [step(params, delta, V, 0.0, int(w_v), int(wx)) for wx in words]
+    vn = np.array([o[0] for o in out])
+    dlx = np.array([o[1] for o in out])
+    assert np.all(vn == vn[0])  # the variance step does not look at the log-price draw
+    k0, k1, k2, k3, k4 = andersen_eq33(params, delta)
+    a = k0 + k1 * V + k2 * vn[0]
+    b2 = k3 * V + k4 * vn[0]
+    A = np.stack([np.ones_like(z), z], axis=1)
+    (fa, fb), *_ = np.linalg.lstsq(A, dlx, rcond=None)
+    scale = abs(a) + abs(fb) * 4.0 + 1e-300
+    ok = (np.max(np.abs(dlx - (fa + fb * z))) <= 1e-12 * scale and abs(fa - a) <= 1e-12 * scale and
+          abs(fb * fb - b2) <= 1e-10 * b2 + 1e-300)
+    return ok, (fa, a, fb * fb, b2, vn[0])
+
+
+# (params, delta, V, w_v): both variance branches, V' far from V (where the split between K1 and
+# K2 matters), V' = 0 from the exponential branch, the stiff benchmark's parameters
+LOG_STEP_CASES = [
+    ([0.04, 0.04, -0.7, 2.0, 0.3], 1 / 52, 0.04, 0x10000000),     # quadratic, Z_V = -1.15: V' < V
+    ([0.04, 0.04, -0.7, 2.0, 0.3], 1 / 52, 0.02, 0xF8000000),     # quadratic, Z_V = +2.0: V' > V
+    ([0.04, 0.04, -0.9, 0.5, 0.9], 1 / 2048, 0.00005, 0x20000000),  # exponential, U = 1/8 < p: V' = 0
+    ([0.04, 0.04, -0.9, 0.5, 0.9], 1 / 52, 0.001, 0xFF000000),     # exponential, U near 1: V' >> V
+    ([0.09, 0.04, 0.5, 3.0, 0.6], 0.25, 0.09, 0x40000000),         # positive rho, long step
+]
+
+
+@pytest.mark.parametrize("params,delta,V,w_v", LOG_STEP_CASES)
+def test_log_step_equals_andersen_eq33_with_central_weights(params, delta, V, w_v):
+    ok, info = check_log_step(O.qe_step, params, delta, V, w_v)
+    assert ok, info
+
+
+def _planted_step(gamma1, gamma2, flip_rho_over_eps=False, drop_k0=False):
+    """A CPU copy of the oracle's step with a planted change in the log-price coefficients (the
+    variance step is the oracle's own, so V' is the same)."""
+    def step(params, delta, V, logX, w_v, w_x):
+        vn, _, _ = O.qe_step(params, delta, V, logX, w_v, w_x)
+        _, theta, rho, kappa, eps = params
+        k0 = 0.0 if drop_k0 else -rho * kappa * theta * delta / eps
+        s = -1.0 if flip_rho_over_eps else 1.0
+        k1 = gamma1 * delta * (kappa * rho / eps - 0.5) - s * rho / eps
+        k2 = gamma2 * delta * (kappa * rho / eps - 0.5) + s * rho / eps
+        k3, k4 = gamma1 * delta * (1 - rho * rho), gamma2 * delta * (1 - rho * rho)
+        z = O.normals_f64([w_x])[0]
+        return vn, logX + k0 + k1 * V + k2 * vn + np.sqrt(k3 * V + k4 * vn) * z
+    return step
+
+
+@pytest.mark.parametrize("planted", ["euler_gamma1", "backward_gamma2", "rho_over_eps_sign", "no_k0"])
+def test_log_step_check_rejects_planted_mistakes(planted):
+    """The check above catches a re-weighting of gamma (which the rho = 0 martingale test and the
+    sigma -> 0 limit cannot see: both hold for any gamma1 + gamma2 = 1), a sign error in the
+    rho / eps terms and a dropped K0, on at least one of the cases; and it passes a correct copy."""
+    step = {"euler_gamma1": _planted_step(1.0, 0.0), "backward_gamma2": _planted_step(0.0, 1.0),
+            "rho_over_eps_sign": _planted_step(0.5, 0.5, flip_rho_over_eps=True),
+            "no_k0": _planted_step(0.5, 0.5, drop_k0=True)}[planted]
+    results = [check_log_step(step, *c, log2_n=8)[0] for c in LOG_STEP_CASES]
+    assert not all(results), planted
+    assert all(check_log_step(_planted_step(0.5, 0.5), *c, log2_n=8)[0] for c in LOG_STEP_CASES)
 
 
 def test_log_step_is_martingale_without_correlation():
