@@ -32,10 +32,27 @@ struct DBuf {
     }
 };
 
+// The synchronous path: the batch's device region comes from the context's resident arena and its
+// staging from the context's pinned buffer. Both grow (once, to the largest call so far) when a
+// call does not fit; a call that fits allocates nothing.
 int run_single(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, double* prices, double* ses) {
     std::string err;
     std::unique_ptr<hexo::Batch> b;
-    int rc = hexo::batch_create(ctx, reqs, n, ctx->stream, nullptr, b, err);
+    size_t need = 0;
+    hexo::BatchMemory mem;
+    mem.arena = &ctx->sync_arena;
+    mem.stage = &ctx->sync_stage;
+    mem.events = false;
+    mem.need = &need;
+    // an arena too small (empty at the first call) reports the size the batch needs
+    int rc = hexo::batch_create(ctx, reqs, n, ctx->stream, mem, b, err);
+    if (rc == HEXO_ERR_NOMEM && need) {
+        // grow; the arena is empty, every earlier call's batch being complete and released
+        const size_t grown = std::max<size_t>({need, (size_t)16 << 20, 2 * ctx->sync_arena.capacity()});
+        if (!ctx->sync_arena.init(grown) && !ctx->sync_arena.init(need))
+            return ctx->set_error(HEXO_ERR_NOMEM, "hipMalloc of " + std::to_string(need) + " device bytes failed");
+        rc = hexo::batch_create(ctx, reqs, n, ctx->stream, mem, b, err);
+    }
     if (rc != HEXO_OK) return ctx->set_error(rc, err);
     rc = hexo::batch_enqueue(*b, PathMode::Normal, false, nullptr, err);
     if (rc == HEXO_OK) rc = hexo::batch_finish(*b, err);
@@ -185,7 +202,7 @@ int hexo_batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hexo_
     auto* hb = new (std::nothrow) hexo_batch();
     if (!hb) return ctx->set_error(HEXO_ERR_NOMEM, "host allocation failed");
     std::string err;
-    int rc = hexo::batch_create(ctx, reqs, n, ctx->stream, nullptr, hb->b, err);
+    int rc = hexo::batch_create(ctx, reqs, n, ctx->stream, hexo::BatchMemory(), hb->b, err);
     if (rc == HEXO_OK) rc = hexo::batch_capture_graph(*hb->b, err);
     if (rc == HEXO_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) {
         rc = HEXO_ERR_HIP;
@@ -309,7 +326,7 @@ int hexo_paths(hexo_ctx* ctx, const hexo_request* req, uint64_t first, uint64_t 
         return ctx->set_error(HEXO_ERR_INVALID, "at most 2^30 finals (paths x expiries x 2) per hexo_paths call");
     std::string err;
     std::unique_ptr<hexo::Batch> b;
-    int rc = hexo::batch_create(ctx, &sub, 1, ctx->stream, nullptr, b, err, &n_fin);
+    int rc = hexo::batch_create(ctx, &sub, 1, ctx->stream, hexo::BatchMemory(), b, err, &n_fin);
     if (rc != HEXO_OK) return ctx->set_error(rc, err);
     double* d_fin = b->pa[0].finals ? b->pa[0].finals : b->pa[1].finals;
     // expiries that are never finalised (trailing chains without strikes) read as NaN
@@ -339,7 +356,7 @@ int hexo_geo_asian_price(hexo_ctx* ctx, const hexo_request* req, double* prices)
     DeviceGuard dg(ctx->device);
     std::string err;
     std::unique_ptr<hexo::Batch> b;
-    int rc = hexo::batch_create(ctx, &q, 1, ctx->stream, nullptr, b, err);
+    int rc = hexo::batch_create(ctx, &q, 1, ctx->stream, hexo::BatchMemory(), b, err);
     if (rc != HEXO_OK) return ctx->set_error(rc, err);
     const uint64_t n_opts = hexo::n_opts_of(&q);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
@@ -443,7 +460,7 @@ int hexo_autotune(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, uint32_t 
                 const hexo_tuning trial{p, b, w, 0};
                 std::string err;
                 std::unique_ptr<hexo::Batch> bt;
-                int rc = hexo::batch_create(ctx, reqs, n, ctx->stream, nullptr, bt, err, nullptr, nullptr, &trial);
+                int rc = hexo::batch_create(ctx, reqs, n, ctx->stream, hexo::BatchMemory(), bt, err, nullptr, &trial);
                 if (rc != HEXO_OK) return ctx->set_error(rc, err);
                 double total = 0.0;
                 for (uint32_t r = 0; r <= reps && rc == HEXO_OK; ++r) {  // first run warms up

@@ -335,14 +335,18 @@ PinnedBuf::~PinnedBuf() {
     if (p) (void)hipHostFree(p);
 }
 
-int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_t stream, Arena* arena,
-                 std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n, PinnedBuf* stage,
+int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_t stream, const BatchMemory& mem,
+                 std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n,
                  const hexo_tuning* tuning) {
     if (n == 0 || !reqs) {
         err = "empty batch";
         return HEXO_ERR_INVALID;
     }
     const hexo_tuning tun = tuning ? *tuning : ctx->tuning_now();
+    // tiles per work item of each kernel class: 1 for the FP64 kernel, the FP32 kernels' paths per
+    // thread; a wave stores one partial per item and output (layout.hpp, kChunkTiles)
+    uint32_t cls_ppt[kPathClasses];
+    for (uint32_t cls = 0; cls < kPathClasses; ++cls) cls_ppt[cls] = cls == 0 ? 1u : tun.paths_per_thread;
     std::vector<ReqPlan> plans(n);
     uint64_t n_seg = 0, n_fin = 0, n_opt = 0, n_partials = 0, n_groups = 0, n_out = 0;
     uint64_t n_geo_items = 0, n_geo_pts = 0, n_geo_waves = 0;
@@ -368,7 +372,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         n_seg += p.segs.size();
         n_fin += p.n_fins;
         n_opt += p.n_opts;
-        n_partials += (uint64_t)p.n_tiles * p.n_out;
+        n_partials += (uint64_t)((p.n_tiles + cls_ppt[p.cls] - 1) / cls_ppt[p.cls]) * p.n_out;
         n_groups += (uint64_t)p.n_groups * p.n_out;
         n_out += p.n_out;
         n_cls[p.cls]++;
@@ -378,22 +382,39 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     b->stream = stream;
     b->n_req = n;
     b->n_out_total = n_out;
-    b->ppt[0] = 1;
-    for (uint32_t cls = 1; cls < kPathClasses; ++cls) b->ppt[cls] = tun.paths_per_thread;
+    for (uint32_t cls = 0; cls < kPathClasses; ++cls) b->ppt[cls] = cls_ppt[cls];
     const uint64_t n_finals = finals_n ? *finals_n : 0;
     const Layout L =
         make_layout(n, n_seg, n_fin, n_opt, n_cls, n_partials, n_groups, n_out, n_finals, n_geo_items, n_geo_pts,
                     n_geo_waves);
 
+    // the device region must fit a fixed region or the arena before anything is staged
+    if ((mem.region && L.total > mem.region_bytes) ||
+        (!mem.region && mem.arena && L.total > mem.arena->capacity() - mem.arena->in_use())) {
+        if (mem.need) *mem.need = L.total;
+        err = "the batch needs " + std::to_string(L.total) + " device bytes; " +
+              (mem.region ? "the fixed region holds " + std::to_string(mem.region_bytes)
+                          : "the arena has " + std::to_string(mem.arena->capacity() - mem.arena->in_use()) + " free");
+        return HEXO_ERR_NOMEM;
+    }
+
     // host staging: the uploaded tables, then the result buffers
     b->upload_bytes = L.upload_end;
     b->h_stage_bytes = L.upload_end + 2 * sizeof(double) * n_out;
-    if (stage) {
-        if (!stage->reserve(b->h_stage_bytes)) {
+    if (mem.host) {
+        if (b->h_stage_bytes > mem.host_bytes) {
+            err = "the batch needs " + std::to_string(b->h_stage_bytes) + " host staging bytes; the fixed buffer holds " +
+                  std::to_string(mem.host_bytes);
+            return HEXO_ERR_NOMEM;
+        }
+        b->h_stage = static_cast<char*>(mem.host);
+        b->h_external = true;
+    } else if (mem.stage) {
+        if (!mem.stage->reserve(b->h_stage_bytes)) {
             err = "hipHostMalloc of the staging buffer failed";
             return HEXO_ERR_NOMEM;
         }
-        b->h_stage = stage->p;
+        b->h_stage = mem.stage->p;
         b->h_external = true;
     } else if (hipHostMalloc(reinterpret_cast<void**>(&b->h_stage), b->h_stage_bytes, hipHostMallocDefault) !=
                hipSuccess) {
@@ -451,6 +472,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         d.n_exp = q.n_expiries;
         d.n_groups = p.n_groups;
         d.v0_f = (float)q.v0;
+        d.store_tiles = cls_ppt[p.cls];
         // options (chain-major) with their expiry's discount factor
         uint64_t oo = 0;
         std::vector<uint32_t> chain_opt_begin(q.n_expiries);
@@ -540,7 +562,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         }
         b->out_base[i] = out_i;
         opt_i += p.n_opts;
-        part_i += (uint64_t)p.n_tiles * p.n_out;
+        part_i += (uint64_t)((p.n_tiles + cls_ppt[p.cls] - 1) / cls_ppt[p.cls]) * p.n_out;
         grp_i += (uint64_t)p.n_groups * p.n_out;
         out_i += p.n_out;
         ge += (uint64_t)p.n_groups * p.n_out;
@@ -583,14 +605,18 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
 
     // device region
     b->d_bytes = L.total;
-    if (arena) {
-        b->d_mem = static_cast<char*>(arena->alloc(L.total));
-        if (!b->d_mem) {
+    if (mem.region) {
+        b->d_mem = static_cast<char*>(mem.region);
+        b->d_external = true;
+    } else if (mem.arena) {
+        b->d_mem = static_cast<char*>(mem.arena->alloc(L.total));
+        if (!b->d_mem) {  // free space, but fragmented
+            if (mem.need) *mem.need = L.total;
             batch_destroy(*b);
             err = "device arena exhausted";
             return HEXO_ERR_NOMEM;
         }
-        b->arena = arena;
+        b->arena = mem.arena;
     } else if (hipMalloc(reinterpret_cast<void**>(&b->d_mem), L.total) != hipSuccess) {
         b->d_mem = nullptr;
         batch_destroy(*b);
@@ -661,7 +687,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         b->block[cls] = block;
     }
     for (auto& e : b->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
+        if (mem.events && hipEventCreate(&e) != hipSuccess) {
             e = nullptr;
             batch_destroy(*b);
             err = "hipEventCreate failed";
@@ -708,6 +734,10 @@ static int enqueue_body(Batch& b, PathMode mode, bool timed, double* d_finals, s
 }
 
 int batch_enqueue(Batch& b, PathMode mode, bool timed, double* d_finals, std::string& err) {
+    if (timed && !b.ev[0]) {
+        err = "the batch was created without timing events";
+        return HEXO_ERR_STATE;
+    }
     b.timed = timed;
     b.t_run = std::chrono::steady_clock::now();
     return enqueue_body(b, mode, timed, d_finals, err);
@@ -778,7 +808,7 @@ void batch_destroy(Batch& b) {
             (void)hipEventDestroy(e);
             e = nullptr;
         }
-    if (b.d_mem) {
+    if (b.d_mem && !b.d_external) {
         if (b.arena)
             b.arena->free(b.d_mem);
         else

@@ -114,6 +114,7 @@ struct Batch {
     char* d_mem = nullptr;
     size_t d_bytes = 0;
     Arena* arena = nullptr;  // owner of d_mem when not hipMalloc'ed
+    bool d_external = false;  // d_mem is a caller's fixed region (freed by nobody here)
     // host pinned staging
     char* h_stage = nullptr;
     size_t h_stage_bytes = 0;
@@ -162,6 +163,13 @@ struct hexo_ctx {
     void* swift_dev = nullptr;
     size_t swift_dev_bytes = 0;
     hexo::PinnedBuf swift_host;
+    // the synchronous pricing calls' device arena and staging buffer, likewise grown on demand and
+    // resident (guarded by mu): hexo_price and hexo_price_batch allocate nothing once they fit
+    hexo::Arena sync_arena;
+    hexo::PinnedBuf sync_stage;
+    // the allocation-free entry point's fixed region on this device (a file-scope device array)
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
     int set_error(int code, const std::string& msg) {
         std::lock_guard<std::mutex> g(err_mu);
         last_error = msg;
@@ -188,12 +196,26 @@ struct hexo_batch {
 
 namespace hexo {
 
+// Where a batch's memory comes from. By default the device region is hipMalloc'ed and the host
+// staging buffer hipHostMalloc'ed per batch (long-lived batches); the serving loop and the
+// synchronous entry points bring a resident arena and staging buffer instead, and the
+// allocation-free entry point a fixed device region and host buffer.
+struct BatchMemory {
+    Arena* arena = nullptr;     // device region sub-allocated from this arena
+    PinnedBuf* stage = nullptr;  // host staging reused from this buffer
+    void* region = nullptr;     // fixed device region of region_bytes (nothing allocated)
+    size_t region_bytes = 0;
+    void* host = nullptr;       // fixed host staging buffer of host_bytes (nothing allocated)
+    size_t host_bytes = 0;
+    bool events = true;         // create the timing events (hexo_batch_run_eager, the autotuner)
+    size_t* need = nullptr;     // out: the device bytes the batch needs, set when they do not fit
+};
+
 // Creates a batch: validates, builds the schedules, packs and uploads the tables, plans the
-// launches. mem/arena: when arena is non-null the device region comes from it.
-// tuning: the launch shape to plan with (null: the context's, read once under tuning_mu).
-int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_t stream, Arena* arena,
+// launches. tuning: the launch shape to plan with (null: the context's, read once under tuning_mu).
+int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_t stream, const BatchMemory& mem,
                  std::unique_ptr<Batch>& out, std::string& err, const uint64_t* finals_n = nullptr,
-                 PinnedBuf* stage = nullptr, const hexo_tuning* tuning = nullptr);
+                 const hexo_tuning* tuning = nullptr);
 // Device bytes a set of requests needs (admission accounting).
 size_t batch_device_bytes(const hexo_request* reqs, uint32_t n);
 size_t request_device_bytes(const hexo_request* r);

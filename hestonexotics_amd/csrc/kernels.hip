@@ -104,6 +104,15 @@ struct TileSums<double> {  // FP64 sums (HEXO_FP32 and the FP64 kernel)
         s += x;
         ss = __builtin_fma(x, x, ss);
     }
+    __device__ __forceinline__ void combine(const TileSums& o) {
+        s += o.s;
+        ss += o.ss;
+    }
+    __device__ __forceinline__ void load(const void* partials, size_t slot) {
+        const double2 v = reinterpret_cast<const double2*>(partials)[slot];
+        s = v.x;
+        ss = v.y;
+    }
     __device__ __forceinline__ void tree8() {
 #pragma unroll
         for (int m = 1; m < 8; m <<= 1) {
@@ -123,6 +132,15 @@ struct TileSums<float> {  // compensated FP32 sums (HEXO_FP32_COMPENSATED)
         s.add(x);
         ss.add(x * x);
     }
+    __device__ __forceinline__ void combine(const TileSums& o) {
+        s.merge(o.s.s, o.s.c);
+        ss.merge(o.ss.s, o.ss.c);
+    }
+    __device__ __forceinline__ void load(const void* partials, size_t slot) {
+        const float4 v = reinterpret_cast<const float4*>(partials)[slot];
+        s = CompF{v.x, v.y};
+        ss = CompF{v.z, v.w};
+    }
     __device__ __forceinline__ void tree8() {
 #pragma unroll
         for (int m = 1; m < 8; m <<= 1) {
@@ -136,6 +154,19 @@ struct TileSums<float> {  // compensated FP32 sums (HEXO_FP32_COMPENSATED)
         reinterpret_cast<float4*>(partials)[slot] = make_float4(s.s, s.c, ss.s, ss.c);
     }
 };
+
+// The level of the chunk tree ((t0 + t1) + (t2 + t3)) that N consecutive tiles of one chunk form
+// (layout.hpp, kChunkTiles): t0 for one tile, t0 + t1 for two, the whole tree for four.
+template <int N, typename Acc>
+__device__ __forceinline__ TileSums<Acc> chunk_level(TileSums<Acc> (&t)[N]) {
+    static_assert(N == 1 || N == 2 || N == kChunkTiles, "tiles per stored partial: 1, 2 or 4");
+    if constexpr (N >= 2) t[0].combine(t[1]);
+    if constexpr (N == 4) {
+        t[2].combine(t[3]);
+        t[0].combine(t[2]);
+    }
+    return t[0];
+}
 
 // Payoff kind of output block `pay` of a request: the pay-th set bit of its payoffs (outputs are
 // payoff-major in bit order: Asian, European, Asian with the geometric control variate, the
@@ -180,15 +211,17 @@ __device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevRe
                 const DevOpt& opt = a.opts[R.opt_begin + F.opt_begin + (active ? kj : 0u)];
                 const KT K = sizeof(KT) == 4 ? (KT)opt.k_rel : (KT)opt.k_abs;
                 const uint32_t o = pay * R.n_opts + F.opt_begin + kj;
+                TileSums<Acc> t[PPT];
 #pragma unroll
                 for (int j = 0; j < PPT; ++j) {
                     const uint32_t tile = tile0 + (uint32_t)j;
-                    if (tile >= R.n_tiles) break;
+                    if (tile >= R.n_tiles) continue;  // absent tiles are zero sums
                     const uint64_t nv = R.n_paths - (uint64_t)tile * kTile;
-                    TileSums<Acc> t;
-                    for (int p = 0; p < kTile; ++p) t.add((Acc)payoff_of(vals[j * kTile + p], kind, K, (uint64_t)p < nv));
-                    if (active) t.store(a.partials, (size_t)R.partial_base + (size_t)tile * n_out + o);
+                    for (int p = 0; p < kTile; ++p)
+                        t[j].add((Acc)payoff_of(vals[j * kTile + p], kind, K, (uint64_t)p < nv));
                 }
+                const TileSums<Acc> c = chunk_level<PPT, Acc>(t);
+                if (active) c.store(a.partials, (size_t)R.partial_base + (size_t)(tile0 / PPT) * n_out + o);
             }
         } else {
             const int g = lane & 7;
@@ -198,20 +231,21 @@ __device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevRe
                 const DevOpt& opt = a.opts[R.opt_begin + F.opt_begin + (active ? kj : 0u)];
                 const KT K = sizeof(KT) == 4 ? (KT)opt.k_rel : (KT)opt.k_abs;
                 const uint32_t o = pay * R.n_opts + F.opt_begin + kj;
+                TileSums<Acc> t[PPT];
 #pragma unroll
                 for (int j = 0; j < PPT; ++j) {
                     const uint32_t tile = tile0 + (uint32_t)j;
-                    if (tile >= R.n_tiles) break;
+                    if (tile >= R.n_tiles) continue;  // absent tiles are zero sums
                     const uint64_t nv = R.n_paths - (uint64_t)tile * kTile;
-                    TileSums<Acc> t;
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
                         const int p = g * 8 + i;
-                        t.add((Acc)payoff_of(vals[j * kTile + p], kind, K, (uint64_t)p < nv));
+                        t[j].add((Acc)payoff_of(vals[j * kTile + p], kind, K, (uint64_t)p < nv));
                     }
-                    t.tree8();
-                    if (g == 0 && active) t.store(a.partials, (size_t)R.partial_base + (size_t)tile * n_out + o);
+                    t[j].tree8();  // every lane of the 8-lane group ends with the same bits
                 }
+                const TileSums<Acc> c = chunk_level<PPT, Acc>(t);
+                if (g == 0 && active) c.store(a.partials, (size_t)R.partial_base + (size_t)(tile0 / PPT) * n_out + o);
             }
         }
     }
@@ -766,6 +800,46 @@ __device__ __forceinline__ uint32_t find_req(const uint64_t* end, uint32_t n, ui
     return lo;
 }
 
+// Stored partial `slot` of output o, or a zero sum past the request's last stored one.
+template <typename Acc>
+__device__ __forceinline__ TileSums<Acc> stored_sum(const ReduceArgs& a, const DevReq& R, uint32_t n_out,
+                                                    uint32_t n_slots, uint32_t slot, uint32_t o) {
+    TileSums<Acc> t;
+    if (slot < n_slots) t.load(a.partials, (size_t)R.partial_base + (size_t)slot * n_out + o);
+    return t;
+}
+
+// Sums the chunks of one 64-tile group in chunk order. Each chunk's sum completes the tree
+// ((t0 + t1) + (t2 + t3)) from the level the path kernel stored (store_tiles tiles per partial),
+// so a chunk's bits do not depend on the paths per thread the request ran with.
+template <typename Acc>
+__device__ __forceinline__ TileSums<Acc> group_sum(const ReduceArgs& a, const DevReq& R, uint32_t n_out, uint32_t g,
+                                                   uint32_t o) {
+    const uint32_t st = R.store_tiles;
+    const uint32_t n_slots = (R.n_tiles + st - 1) / st;
+    const uint32_t t0 = g * kGroupTiles;
+    const uint32_t t1 = min(t0 + (uint32_t)kGroupTiles, R.n_tiles);
+    TileSums<Acc> acc;
+    for (uint32_t c = t0 / kChunkTiles; c < (t1 + kChunkTiles - 1) / kChunkTiles; ++c) {
+        TileSums<Acc> chunk;
+        if (st == kChunkTiles) {
+            chunk = stored_sum<Acc>(a, R, n_out, n_slots, c, o);
+        } else if (st == 2) {
+            TileSums<Acc> t[2] = {stored_sum<Acc>(a, R, n_out, n_slots, 2 * c, o),
+                                  stored_sum<Acc>(a, R, n_out, n_slots, 2 * c + 1, o)};
+            t[0].combine(t[1]);
+            chunk = t[0];
+        } else {
+            TileSums<Acc> t[kChunkTiles];
+#pragma unroll
+            for (int i = 0; i < kChunkTiles; ++i) t[i] = stored_sum<Acc>(a, R, n_out, n_slots, kChunkTiles * c + i, o);
+            chunk = chunk_level<kChunkTiles, Acc>(t);
+        }
+        acc.combine(chunk);
+    }
+    return acc;
+}
+
 __global__ __launch_bounds__(256) void k_reduce_groups(ReduceArgs a, uint64_t total) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
@@ -775,28 +849,11 @@ __global__ __launch_bounds__(256) void k_reduce_groups(ReduceArgs a, uint64_t to
     const uint32_t n_out = R.n_opts * R.n_pay;
     const uint32_t g = (uint32_t)(local / n_out);
     const uint32_t o = (uint32_t)(local - (uint64_t)g * n_out);
-    const uint32_t t0 = g * kGroupTiles;
-    const uint32_t t1 = min(t0 + (uint32_t)kGroupTiles, R.n_tiles);
     const size_t gslot = (size_t)R.group_base + (size_t)g * n_out + o;
-    if (R.precision == kPrecComp) {
-        const float4* p = reinterpret_cast<const float4*>(a.partials);
-        CompF s{0.f, 0.f}, ss{0.f, 0.f};
-        for (uint32_t tile = t0; tile < t1; ++tile) {
-            const float4 v = p[(size_t)R.partial_base + (size_t)tile * n_out + o];
-            s.merge(v.x, v.y);
-            ss.merge(v.z, v.w);
-        }
-        reinterpret_cast<float4*>(a.groups)[gslot] = make_float4(s.s, s.c, ss.s, ss.c);
-    } else {
-        const double2* p = reinterpret_cast<const double2*>(a.partials);
-        double s = 0.0, ss = 0.0;
-        for (uint32_t tile = t0; tile < t1; ++tile) {
-            const double2 v = p[(size_t)R.partial_base + (size_t)tile * n_out + o];
-            s += v.x;
-            ss += v.y;
-        }
-        reinterpret_cast<double2*>(a.groups)[gslot] = make_double2(s, ss);
-    }
+    if (R.precision == kPrecComp)
+        group_sum<float>(a, R, n_out, g, o).store(a.groups, gslot);
+    else
+        group_sum<double>(a, R, n_out, g, o).store(a.groups, gslot);
 }
 
 // Sum and sum of squares of output o over the request's groups, on one wavefront: lane l sums

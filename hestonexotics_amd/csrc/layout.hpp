@@ -6,8 +6,9 @@
 //   DevFin[n_fin]        expiry finalisations at the end of a run
 //   DevOpt[n_opt]        strikes (relative and absolute) and discount factor per option
 //   uint32 order[n_req]  requests sorted longest-first; item_end[n_req] their work-item prefix sums
-//   partials             16 bytes per (64-path tile, output): FP64 (sum, sum of squares) or
-//                        compensated FP32 (sum, carry, sum of squares, carry)
+//   partials             16 bytes per (stored tile sum, output): FP64 (sum, sum of squares) or
+//                        compensated FP32 (sum, carry, sum of squares, carry). A path-kernel wave
+//                        that holds several tiles of one 4-tile chunk combines them first (below)
 //   groups, results      per (64-tile group, output) partials; per output (price, std error)
 #pragma once
 #include <stdint.h>
@@ -16,6 +17,12 @@ namespace hexo {
 
 constexpr int kTile = 64;         // paths per tile = one wavefront
 constexpr int kGroupTiles = 64;   // tiles per first-level reduction group
+// Tile sums are combined in chunks of kChunkTiles consecutive tiles by one fixed pairwise tree,
+// ((t0 + t1) + (t2 + t3)), with absent tiles (beyond the request's last) taken as zero sums. A
+// path-kernel wave holding s = 1, 2 or 4 tiles of a chunk forms the tree's level for s tiles and
+// stores that (the request's store_tiles); the group reduction completes the tree. The sums are
+// therefore the same, bit for bit, whatever the paths per thread of the launch.
+constexpr int kChunkTiles = 4;
 constexpr int kMaxPPT = 4;        // paths per thread (FP32 kernel)
 constexpr unsigned long long kMaxPaths = (1ull << 37) - kTile;  // per request: 2^31 - 1 tiles
 
@@ -78,7 +85,8 @@ struct DevReq {
     uint32_t n_opts, n_pay, payoffs, flags;
     uint32_t precision, n_tiles, opt_begin, out_base;
     uint32_t n_exp, n_groups;
-    float v0_f, pad0;
+    float v0_f;
+    uint32_t store_tiles;  // tiles per stored partial: 1, 2 or kChunkTiles (the kernel's paths per thread)
 };
 
 struct PathArgs {

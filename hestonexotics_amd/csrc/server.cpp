@@ -184,8 +184,11 @@ void hexo_server_state::launch_ready() {
         hexo_tuning tun = ctx->tuning_now();
         if (large && (tun.waves_per_simd == 0 || tun.waves_per_simd > kLargeWavesPerSimd))
             tun.waves_per_simd = kLargeWavesPerSimd;
-        int rc = hexo::batch_create(ctx, reqs.data(), (uint32_t)reqs.size(), streams[slot], &arena, b, err, nullptr,
-                                    stages[slot].get(), &tun);
+        hexo::BatchMemory mem;
+        mem.arena = &arena;
+        mem.stage = stages[slot].get();
+        mem.events = false;
+        int rc = hexo::batch_create(ctx, reqs.data(), (uint32_t)reqs.size(), streams[slot], mem, b, err, nullptr, &tun);
         if (rc == HEXO_ERR_NOMEM && !flights.empty()) {
             // fragmentation: put the requests back in arrival order and retry when a batch completes
             for (uint64_t id : ids) queue.push_back(id);
