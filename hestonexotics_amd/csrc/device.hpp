@@ -139,14 +139,20 @@ __device__ __forceinline__ float hornerf(const float (&c)[N], float r) {
 // N(0,1) in FP32 from a 32-bit word through AS241 PPND7 (the FP32 kernel's inverse normal).
 // q = (k - 2^31 + 1/2) 2^-32 and the tail argument min(u, 1-u) = (min(k, ~k) + 1/2) 2^-32 are
 // formed from the integer, so both tails keep full relative precision (no 1 - u cancellation).
-// Central region: returns q A(r)/B(r) and sets `central` = |q| <= 0.425 (the value is meaningless
-// when `central` is false).
-__device__ __forceinline__ float normal_central_f32(uint32_t k, bool& central) {
+__device__ __forceinline__ float central_q_f32(uint32_t k) {
     const int32_t sk = (int32_t)(k ^ 0x80000000u);
-    const float q = __builtin_fmaf((float)sk, 0x1p-32f, 0x1p-33f);
+    return __builtin_fmaf((float)sk, 0x1p-32f, 0x1p-33f);
+}
+// Central region: q A(r)/B(r) with r = 0.180625 - q^2, meaningful for |q| <= 0.425.
+__device__ __forceinline__ float normal_central_rational_f32(float q) {
     const float rc = __builtin_fmaf(-q, q, 0.180625f);
-    central = __builtin_fabsf(q) <= 0.425f;
     return (q * hornerf(Ppnd7::a, rc)) * __builtin_amdgcn_rcpf(hornerf(Ppnd7::b, rc));
+}
+// The central value of word k; `central` = |q| <= 0.425 (the value is meaningless otherwise).
+__device__ __forceinline__ float normal_central_f32(uint32_t k, bool& central) {
+    const float q = central_q_f32(k);
+    central = __builtin_fabsf(q) <= 0.425f;
+    return normal_central_rational_f32(q);
 }
 // Tail region (|q| > 0.425): r = sqrt(-ln(min(u, 1-u))) - 1.6, C(r)/D(r) with the sign of q.
 __device__ __forceinline__ float normal_tail_f32(uint32_t k) {

@@ -301,7 +301,7 @@ size_t request_device_bytes(const hexo_request* r) {
     if (!plan_request(r, p)) return 0;
     return sizeof(DevReq) + p.segs.size() * sizeof(DevSeg) + p.n_fins * sizeof(DevFin) + p.n_opts * sizeof(DevOpt) +
            32 + 16 + (size_t)16 * p.n_tiles * p.n_out + (size_t)16 * p.n_groups * p.n_out + 24 * p.n_out +
-           p.geo.items.size() * (sizeof(DevGeo) + 4 + (kGeoMaxP0 + 1) * 520 + 8 * 67) + p.geo.pts.size() * sizeof(double);
+           p.geo.items.size() * (sizeof(DevGeo) + 4 + (kGeoMaxP0 + 1) * 520 + sizeof(double) * geo_state_doubles(1)) + p.geo.pts.size() * sizeof(double);
 }
 
 double request_work(const hexo_request* r) {
@@ -495,8 +495,9 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
             g.omd = (float)omd;
             g.ope = (float)(1.0 + dd);
             g.th = (float)th;
-            g.c1 = (float)(e * e * dd * omd / k);
-            g.c2 = (float)(th * e * e * omd * omd / (2.0 * k));
+            // half the conditional variance's coefficients (the FP32 step forms s2 / 2)
+            g.c1h = (float)(0.5 * e * e * dd * omd / k);
+            g.c2h = (float)(th * e * e * omd * omd / (4.0 * k));
             const double kc = 0.5 * dl * (k * rho / e - 0.5);
             const double kd = rho / e;
             const double K3 = 0.5 * dl * (1.0 - rho * rho);
@@ -707,6 +708,34 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         batch_destroy(*b);
         err = "launch of the geometric control-variate prices failed";
         return HEXO_ERR_HIP;
+    }
+    if (ga.n_items) {
+        // A request whose exact price did not converge (NaN) is refused rather than priced NaN: its
+        // variance process stays near 0 (v0 = 0 with a Feller ratio 2 kappa theta / sigma^2 near 0),
+        // so the characteristic function of the geometric average's log hardly decays. The result
+        // buffer is free until the first run.
+        if (hipMemcpyAsync(b->h_prices, ga.geo, sizeof(double) * n_out, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            (void)hipStreamSynchronize(stream);
+            batch_destroy(*b);
+            err = "read-back of the geometric control-variate prices failed";
+            return HEXO_ERR_HIP;
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!(reqs[i].payoffs & HEXO_ASIAN_GEO_CV)) continue;
+            const uint64_t first = b->out_base[i] + n_payoffs(reqs[i].payoffs & (HEXO_ASIAN | HEXO_EUROPEAN)) * plans[i].n_opts;
+            for (uint64_t o = first; o < first + plans[i].n_opts; ++o)
+                if (std::isnan(b->h_prices[o])) {
+                    const hexo_request& q = reqs[i];
+                    err = "request " + std::to_string(i) +
+                          ": the geometric control variate's exact price did not converge (v0 " + std::to_string(q.v0) +
+                          ", Feller ratio 2 kappa theta / sigma^2 = " +
+                          std::to_string(2.0 * q.kappa * q.theta / (q.sigma * q.sigma)) +
+                          ": the variance stays near 0); price it without HEXO_ASIAN_GEO_CV";
+                    batch_destroy(*b);
+                    return HEXO_ERR_INVALID;
+                }
+        }
     }
     out = std::move(b);
     return HEXO_OK;

@@ -136,8 +136,9 @@ __device__ __forceinline__ void strike_of(const GeoArgs& a, const DevGeo& G, int
 }
 
 // Per-item state between rounds (kGeoState doubles): each strike's integral so far, E[e^L], the
-// panels summed so far and whether the sum has converged.
-constexpr int kGeoState = 64 + 3;
+// panels summed so far, whether the sum has converged and whether the last panel summed was below
+// the stop rule's bound.
+constexpr int kGeoState = 64 + 4;
 constexpr int kGeoRounds = 3;  // parallel rounds of p0 panels before the serial continuation
 
 // Pass 1 of round r: every wavefront evaluates one panel of one unconverged item (round r covers
@@ -173,10 +174,12 @@ __global__ __launch_bounds__(256) void k_geo_panels(GeoArgs a, uint32_t round) {
 }
 
 // Pass 2 of round r: one wavefront per item; lane i adds strike i's panel sums in panel order up
-// to the first panel (after the first) whose |phi| stays below 1e-15 u0, u0 its start: beyond it the
-// integrand |phi| / (u^2 + 1/4) leaves at most 1e-15 to the integral. An unconverged item keeps
-// its sums for the next round; in the last round the wavefront evaluates further panels itself.
-// The price is the same ordered sum whatever p0 and the number of rounds are.
+// to the second of the first two consecutive panels (after the first) whose |phi| stays below
+// 1e-15 u0, u0 a panel's start: beyond u0 the integrand |phi| / (u^2 + 1/4) leaves at most
+// sup|phi| / u0 to the integral, and two panels in a row guard against a dip of a non-monotone
+// |phi|. An unconverged item keeps its sums for the next round; in the last round the wavefront
+// evaluates further panels itself. The price is the same ordered sum whatever p0 and the number of
+// rounds are.
 __global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a, uint32_t round) {
     __shared__ double sh[kGeoWaves][3][64];
     const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
@@ -193,11 +196,14 @@ __global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a, uint32_t round) {
     const double e1 = round == 0 ? a.panel_max[G.wave_begin + G.p0] : st[64];  // E[e^L]
     double integ = round == 0 ? 0.0 : st[lane];
     const uint32_t base = round == 0 ? 0u : (uint32_t)st[65];
+    bool below = round == 0 ? false : st[67] != 0.0;  // the previous panel was below the bound
     bool converged = false;
     uint32_t p = base;
     for (uint32_t j = 0; j < G.p0 && p < G.max_panels && !converged; ++j, ++p) {
         integ += a.panel[(size_t)(G.wave_begin + j) * 64 + (uint32_t)lane];
-        converged = p > 0 && a.panel_max[G.wave_begin + j] < 1e-15 * (double)p * G.h;
+        const bool now = p > 0 && a.panel_max[G.wave_begin + j] < 1e-15 * (double)p * G.h;
+        converged = now && below;
+        below = now;
     }
     const bool last = round + 1 == kGeoRounds;
     if (!converged && !last && p < G.max_panels) {
@@ -206,13 +212,16 @@ __global__ __launch_bounds__(256) void k_geo_sum(GeoArgs a, uint32_t round) {
             st[64] = e1;
             st[65] = (double)p;
             st[66] = 0.0;
+            st[67] = below ? 1.0 : 0.0;
         }
         return;
     }
     for (; p < G.max_panels && !converged; ++p) {
         double mag;
         integ += panel_sum(a, G, R, pts, p, lane, k, has_k, sh[w], &mag);
-        converged = p > 0 && mag < 1e-15 * (double)p * G.h;  // wave-uniform
+        const bool now = p > 0 && mag < 1e-15 * (double)p * G.h;  // wave-uniform
+        converged = now && below;
+        below = now;
     }
     if (lane == 0) st[66] = 1.0;  // later rounds skip the item
     if ((uint32_t)lane >= G.n_strikes) return;

@@ -281,7 +281,7 @@ __device__ __forceinline__ void count_flush(unsigned long long* counts, uint32_t
 // FP32 path kernel
 // ------------------------------------------------------------------------------------------
 struct SegF {
-    float omd, th, c1, c2, g0, kc, kd, k3;
+    float omd, th, c1h, c2h, g0, kc, kd, k3;
     float mA, ma, mh, mr;
     float ope;
 };
@@ -294,35 +294,36 @@ struct WaveLds {
 };
 
 // AS241 tail evaluations gathered across the wave. Each lane holds N normals whose central-region
-// value is already in z[s]; where t[s] is set (|q| > 0.425) the tail rational is needed instead,
-// and mask[s] is the wave's ballot of t[s]. The wave packs those words into an LDS queue (the
-// ballot and mbcnt give each one a slot), evaluates
-// the queue densely 64 entries at a time, and each lane reads its results back. With 15% tail
-// draws this replaces N full-wave tail evaluations by ceil(0.15 * 64 N / 64) dense ones.
+// value is already in z[s]; where t[s] is set (|q| > 0.425) the tail rational is needed instead.
+// The wave packs those words into an LDS queue, evaluates the queue densely 64 entries at a time,
+// and each lane reads its results back. With 15% tail draws this replaces N full-wave tail
+// evaluations by ceil(0.15 * 64 N / 64) dense ones. A word's slot is the wave-uniform count of
+// earlier entries plus its lane's rank among the wave's lanes with a tail draw (mbcnt over the
+// ballot); the uniform part stays in a scalar register and enters the LDS address as a base, so
+// a slot costs three vector instructions (two mbcnt, one shift-add), and its result is read back
+// from the same address at a fixed offset. q must be wave-uniform.
 template <int N>
-__device__ __forceinline__ void tail_normals(const uint32_t (&w)[N], const bool (&t)[N], const uint64_t (&mask)[N],
+__device__ __forceinline__ void tail_normals(const uint32_t (&w)[N], const bool (&t)[N], const uint64_t (&mk)[N],
                                              float (&z)[N], uint32_t* q, int lane) {
-    uint32_t* q_in = q;
-    float* q_out = reinterpret_cast<float*>(q + N * kTile);
-    uint32_t slot[N];
+    uint32_t rank[N], base[N];
     uint32_t total = 0;  // wave-uniform
 #pragma unroll
     for (int s = 0; s < N; ++s) {
-        // slot = total + number of lower lanes with a tail draw (mbcnt adds its accumulator)
-        slot[s] = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[s] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask[s], total));
-        if (t[s]) q_in[slot[s]] = w[s];
-        total += (uint32_t)__builtin_popcountll(mask[s]);
+        rank[s] = __builtin_amdgcn_mbcnt_hi((uint32_t)(mk[s] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[s], 0u));
+        base[s] = total;
+        if (t[s]) q[base[s] + rank[s]] = w[s];
+        total += (uint32_t)__builtin_popcountll(mk[s]);
     }
-    total = uniform(total);
+    float* q_out = reinterpret_cast<float*>(q + N * kTile);
     wave_sync_lds();
     for (uint32_t p = 0; p < total; p += kTile) {
         const uint32_t i = p + (uint32_t)lane;
-        if (i < total) q_out[i] = normal_tail_f32(q_in[i]);
+        if (i < total) q_out[i] = normal_tail_f32(q[i]);
     }
     wave_sync_lds();
 #pragma unroll
     for (int s = 0; s < N; ++s)
-        if (t[s]) z[s] = q_out[slot[s]];
+        if (t[s]) z[s] = q_out[base[s] + rank[s]];
     wave_sync_lds();  // the queue is rewritten by the next step
 }
 
@@ -331,19 +332,22 @@ struct StepCount {
 };
 
 // The step's conditional moments from V (src/HSimulation.tpp:58-62): d0 = th - V, m = V + omd d0
-// (exactly th at V = th, whatever the roundings), s2, m^2 and the branch (quadratic when
-// psi = s2/m^2 < 1.5).
+// (exactly th at V = th, whatever the roundings), s2h = s2 / 2 (from the halved constants: the
+// same bits as s2, scaled), m^2, tt = 2 / psi = m^2 / s2h and the branch: quadratic when
+// psi < 1.5, i.e. tt > 4/3. tt is formed before the branch because the quadratic branch starts
+// from it; the exponential branch does not use it.
 template <int PPT>
 __device__ __forceinline__ void qe_moments_fp32(const float (&V)[PPT], const SegF& c, float (&d0)[PPT],
-                                                float (&m)[PPT], float (&s2)[PPT], float (&m2)[PPT],
-                                                bool (&quad)[PPT]) {
+                                                float (&m)[PPT], float (&s2h)[PPT], float (&m2)[PPT],
+                                                float (&tt)[PPT], bool (&quad)[PPT]) {
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         d0[j] = c.th - V[j];
         m[j] = __builtin_fmaf(c.omd, d0[j], V[j]);
-        s2[j] = __builtin_fmaf(V[j], c.c1, c.c2);
+        s2h[j] = __builtin_fmaf(V[j], c.c1h, c.c2h);
         m2[j] = m[j] * m[j];
-        quad[j] = s2[j] < 1.5f * m2[j];
+        tt[j] = m2[j] * __builtin_amdgcn_rcpf(s2h[j]);  // finite by the FP32 validation bound
+        quad[j] = tt[j] > 4.0f / 3.0f;
     }
 }
 
@@ -356,8 +360,8 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
                                                float (&P)[PPT], float (&Yp)[PPT], float (&PY)[PPT],
                                                const uint32_t (&wv)[PPT], const float (&zv)[PPT],
                                                const float (&zx)[PPT], const float (&d0)[PPT], const float (&m)[PPT],
-                                               const float (&s2)[PPT], const float (&m2)[PPT], const bool (&quad)[PPT],
-                                               const SegF& c) {
+                                               const float (&s2h)[PPT], const float (&m2)[PPT], const float (&tt)[PPT],
+                                               const bool (&quad)[PPT], const SegF& c) {
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         float dm;  // V' - m
@@ -366,10 +370,8 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
         float ft = 0.0f, fs = 0.0f, corr = 0.0f;
         bool mok = false;
         if (quad[j]) {
-            const float inv = __builtin_amdgcn_rcpf(s2[j]);
-            const float tt = (m2[j] + m2[j]) * inv;  // 2 / psi (finite by the FP32 validation bound)
-            const float tm1 = tt - 1.0f;
-            const float b2 = tm1 + __builtin_amdgcn_sqrtf(tt * tm1);
+            const float tm1 = tt[j] - 1.0f;
+            const float b2 = tm1 + __builtin_amdgcn_sqrtf(tt[j] * tm1);
             const float b = __builtin_amdgcn_sqrtf(b2);
             // V' = a (b + Z)^2 with a = m / (1 + b^2), taken as V' - m = a (Z (Z + 2 b) - 1): its
             // mean is 0 whatever the rounding of a and b, so the scheme's E[V'] = m holds exactly
@@ -389,7 +391,7 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
         } else {
             // p = (psi-1)/(psi+1), beta = 2/(m (psi+1)); V' = ln((1-p)/(1-U))/beta if U > p, else 0.
             // With sm = s2 + m^2: 1-p = 2m^2/sm and 1/beta = sm/(2m).
-            const float sm = s2[j] + m2[j];
+            const float sm = __builtin_fmaf(2.0f, s2h[j], m2[j]);
             const float lhs = one_minus_uniform_f32(wv[j]) * sm;  // (1-U) sm
             const float tm2 = m2[j] + m2[j];
             const float lg = __builtin_amdgcn_logf(tm2 * rcp_refined(lhs));
@@ -444,29 +446,34 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
 }
 
 
-// Appends the normals of one step for the tail queue: slots s0 (Z_V) and s0 + 1 (Z_X) of path j.
-// need_v: Z_V is needed for sure (its step's branch is known to be quadratic) or possibly.
+// The normals of one step of path j, in queue slots s0 (Z_V) and s0 + 1 (Z_X): their central-region
+// values and whether the tail rational is needed instead (for Z_V only where the step's variance
+// branch is quadratic: need_v). tv / tx: the words' own tail tests (branch counts).
+// mk: the wave's ballots of t, formed from the ballots of the comparisons themselves (the ballot of
+// a combined bool goes through a vector register: two more instructions).
 template <int N>
 __device__ __forceinline__ void queue_normals(uint32_t wv, uint32_t wx, bool need_v, int s0, float (&z)[N],
-                                              uint32_t (&w)[N], bool (&t)[N], uint64_t (&mk)[N], bool& cv,
-                                              bool& cx) {
-    z[s0] = normal_central_f32(wv, cv);
-    z[s0 + 1] = normal_central_f32(wx, cx);
+                                              uint32_t (&w)[N], bool (&t)[N], uint64_t (&mk)[N], bool& tv,
+                                              bool& tx) {
+    const float qv = central_q_f32(wv), qx = central_q_f32(wx);
+    z[s0] = normal_central_rational_f32(qv);
+    z[s0 + 1] = normal_central_rational_f32(qx);
     w[s0] = wv;
     w[s0 + 1] = wx;
-    t[s0] = need_v && !cv;
-    t[s0 + 1] = !cx;
-    // ballots of the comparisons themselves (a ballot of the combined bool costs two VALU ops)
-    mk[s0] = __builtin_amdgcn_ballot_w64(need_v) & __builtin_amdgcn_ballot_w64(!cv);
-    mk[s0 + 1] = __builtin_amdgcn_ballot_w64(!cx);
+    tv = __builtin_fabsf(qv) > 0.425f;
+    tx = __builtin_fabsf(qx) > 0.425f;
+    t[s0] = need_v && tv;
+    t[s0 + 1] = tx;
+    mk[s0] = __builtin_amdgcn_ballot_w64(need_v) & __builtin_amdgcn_ballot_w64(tv);
+    mk[s0 + 1] = __builtin_amdgcn_ballot_w64(tx);
 }
 
-__device__ __forceinline__ void count_step(StepCount& cnt, bool valid, bool quad, bool cv, bool cx) {
+__device__ __forceinline__ void count_step(StepCount& cnt, bool valid, bool quad, bool tv, bool tx) {
     if (!valid) return;
     cnt.quad += quad ? 1u : 0u;
     cnt.exp += quad ? 0u : 1u;
-    cnt.central += (quad && cv ? 1u : 0u) + (cx ? 1u : 0u);
-    cnt.tail += (quad && !cv ? 1u : 0u) + (cx ? 0u : 1u);
+    cnt.central += (quad && !tv ? 1u : 0u) + (tx ? 0u : 1u);
+    cnt.tail += (quad && tv ? 1u : 0u) + (tx ? 1u : 0u);
 }
 
 // One QE step for the PPT paths of a lane (Andersen's QE scheme as in src/HSimulation.tpp:52-86):
@@ -478,18 +485,18 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
                                              const uint32_t (&wv)[PPT], const uint32_t (&wx)[PPT],
                                              const SegF& c, uint32_t* q, int lane, const bool (&valid)[PPT],
                                              StepCount& cnt) {
-    float d0[PPT], m[PPT], s2[PPT], m2[PPT], zv[PPT], zx[PPT];
+    float d0[PPT], m[PPT], s2h[PPT], m2[PPT], tt[PPT], zv[PPT], zx[PPT];
     bool quad[PPT];
     uint32_t w[2 * PPT];
     bool t[2 * PPT];
     uint64_t mk[2 * PPT];
     float z[2 * PPT];
-    qe_moments_fp32<PPT>(V, c, d0, m, s2, m2, quad);
+    qe_moments_fp32<PPT>(V, c, d0, m, s2h, m2, tt, quad);
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        bool cv, cx;
-        queue_normals<2 * PPT>(wv[j], wx[j], quad[j], 2 * j, z, w, t, mk, cv, cx);
-        if constexpr (COUNT) count_step(cnt, valid[j], quad[j], cv, cx);
+        bool tv, tx;
+        queue_normals<2 * PPT>(wv[j], wx[j], quad[j], 2 * j, z, w, t, mk, tv, tx);
+        if constexpr (COUNT) count_step(cnt, valid[j], quad[j], tv, tx);
     }
     tail_normals<2 * PPT>(w, t, mk, z, q, lane);
 #pragma unroll
@@ -497,7 +504,7 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
         zv[j] = z[2 * j];
         zx[j] = z[2 * j + 1];
     }
-    qe_update_fp32<PPT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, wv, zv, zx, d0, m, s2, m2, quad, c);
+    qe_update_fp32<PPT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, wv, zv, zx, d0, m, s2h, m2, tt, quad, c);
 }
 
 template <int PPT>
@@ -547,7 +554,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
     const uint32_t n_segs = R.n_segs;
     for (uint32_t s = 0; s < n_segs; ++s) {
         const DevSeg& S = a.segs[R.seg_begin + s];
-        const SegF c{S.omd, S.th, S.c1, S.c2, S.g0, S.kc, S.kd, S.k3, S.mA, S.ma, S.mh, S.mr, S.ope};
+        const SegF c{S.omd, S.th, S.c1h, S.c2h, S.g0, S.kc, S.kd, S.k3, S.mA, S.ma, S.mh, S.mr, S.ope};
         const uint32_t n = S.n_steps;
         uint32_t i = 0;
         if ((k & 1u) && n > 0) {  // second half of a block opened by the previous run
@@ -633,7 +640,8 @@ template <int PPT, bool DEBUG, bool COUNT, bool MART, bool GEO>
 __global__ __launch_bounds__(256) void k_path_fp32(PathArgs a) {
     extern __shared__ uint32_t lds_u32[];
     const int lane = (int)(threadIdx.x & 63u);
-    uint32_t* lds = lds_u32 + (threadIdx.x >> 6) * WaveLds<PPT>::kWords;
+    // the wave's LDS region, at a wave-uniform address
+    uint32_t* lds = lds_u32 + uniform(threadIdx.x >> 6) * WaveLds<PPT>::kWords;
     for (;;) {
         const uint32_t item = next_item(a.counter, lane);
         if (item >= a.total_items) break;

@@ -29,7 +29,7 @@ constexpr unsigned long long kMaxPaths = (1ull << 37) - kTile;  // per request: 
 // One run of steps with a constant step width (the reference's dt = T_c / steps between two
 // expiry finalisations). FP32 constants are computed on the host in FP64 (per request, per run):
 //   m  = V + omd (th - V)            (conditional mean, src/HSimulation.tpp:58-59; omd = 1 - e^-kappa dt)
-//   s2 = V c1 + c2                   (conditional variance, src/HSimulation.tpp:60)
+//   s2 = 2 (V c1h + c2h)             (conditional variance, src/HSimulation.tpp:60; halved constants)
 //   y += g0 + kc (V + V' - 2 th) + kd (V' - V) + sqrt(k3 (V + V')) Z      (log2 of X/S0)
 // with kc = (K1 + K2)/2, kd = (K2 - K1)/2 = rho/eps, k3 = K3 = K4 and g0 = K0 + 2 th kc
 // (src/HSimulation.tpp:75-80), all scaled by log2(e) (k3 by log2(e)^2); th is theta rounded to
@@ -44,7 +44,7 @@ constexpr unsigned long long kMaxPaths = (1ull << 37) - kTile;  // per request: 
 // by construction, so the first term has mean zero whatever the rounding of ma, and
 // A m - ln E[exp(A V')] is O((A a)^2), evaluated in forms without cancellation.
 struct DevSeg {
-    float omd, th, c1, c2;
+    float omd, th, c1h, c2h;
     float g0, kc, kd, k3;
     float mA, ma, mh, mr;  // martingale-corrected step (HEXO_MARTINGALE)
     float delta_f;  // Asian weight of accumulations inside the run

@@ -165,8 +165,12 @@ static void gauss_legendre(int n, double* x, double* w) {
 #define ORC_GL 64
 #define ORC_MAX_PANELS 20000
 
-int orc_geo_price_weights(const double p[5], double S0, double r, const double* t, const double* w, int64_t n,
-                          uint32_t n_strikes, const double* strikes, double* out) {
+/* Stop rule: beyond u0 = pnl h the integrand |phi| / (u^2 + 1/4) leaves at most sup|phi| / u0 to
+ * the integral. The sum stops after the first two consecutive panels (after the first) whose
+ * largest |phi| is below tol u0; tol = 1e-15 in orc_geo_price_weights. Two panels, not one: |phi|
+ * need not fall monotonically, and one panel can sit in a dip. */
+int orc_geo_price_weights_tol(const double p[5], double S0, double r, const double* t, const double* w, int64_t n,
+                              uint32_t n_strikes, const double* strikes, double tol, double* out) {
     if (n < 1) return -1;
     double W = 0.0;
     for (int64_t j = 0; j <= n; ++j) W += w[j];
@@ -204,7 +208,7 @@ int orc_geo_price_weights(const double p[5], double S0, double r, const double* 
     double gx[ORC_GL], gw[ORC_GL];
     gauss_legendre(ORC_GL, gx, gw);
     const double complex e1 = orc_geo_cf(p, r, t, v, n, 1.0);
-    int converged = 0;
+    int converged = 0, below = 0;
     /* panel 0 holds the integrand's 1/(u^2 + 1/4) peak: min(64, ceil(h/16)) sub-panels of 64 nodes
        (64 nodes over [0, 16] integrate it to 1e-13, over [0, 120] only to 1e-4) */
     const int n_first = (int)fmin(64.0, ceil(h / 16.0));
@@ -221,8 +225,9 @@ int orc_geo_price_weights(const double p[5], double S0, double r, const double* 
                 for (uint32_t i = 0; i < n_strikes; ++i)
                     if (isfinite(kk[i])) integ[i] += wq * creal(cexp(I * u * kk[i]) * ph) / (u * u + 0.25);
             }
-        /* beyond u0 = pnl h the integrand |phi|/(u^2 + 1/4) leaves at most sup|phi| / u0 */
-        if (pnl > 0 && amax < 1e-15 * pnl * h) converged = 1;
+        const int now_below = pnl > 0 && amax < tol * pnl * h;
+        converged = now_below && below;
+        below = now_below;
     }
     for (uint32_t i = 0; i < n_strikes; ++i)
         out[i] = !converged ? NAN
@@ -232,6 +237,11 @@ int orc_geo_price_weights(const double p[5], double S0, double r, const double* 
     free(kk);
     free(integ);
     return converged ? 0 : -4;
+}
+
+int orc_geo_price_weights(const double p[5], double S0, double r, const double* t, const double* w, int64_t n,
+                          uint32_t n_strikes, const double* strikes, double* out) {
+    return orc_geo_price_weights_tol(p, S0, r, t, w, n, n_strikes, strikes, 1e-15, out);
 }
 
 int orc_geo_asian_price(const double params[5], double S0, double r, const double* T, uint32_t n_exp,

@@ -113,6 +113,9 @@ void orc_geo_cf_out(const double params[5], double r, const double* t, const dou
                     double s_im, double out[2]);
 int orc_geo_price_weights(const double params[5], double S0, double r, const double* t, const double* w, int64_t n,
                           uint32_t n_strikes, const double* strikes, double* out);
+/* the same with the stop rule's tolerance as an argument (orc_geo_price_weights: 1e-15) */
+int orc_geo_price_weights_tol(const double params[5], double S0, double r, const double* t, const double* w, int64_t n,
+                              uint32_t n_strikes, const double* strikes, double tol, double* out);
 int orc_geo_asian_price(const double params[5], double S0, double r, const double* T, uint32_t n_exp,
                         const uint32_t* n_strikes, uint32_t steps, uint32_t flags, const double* strikes,
                         double* out);

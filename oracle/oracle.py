@@ -56,6 +56,9 @@ def _bind(lib: C.CDLL) -> C.CDLL:
     lib.orc_geo_price_weights.argtypes = [_f64p, C.c_double, C.c_double, _f64p, _f64p, C.c_int64, C.c_uint32,
                                           _f64p, _f64p]
     lib.orc_geo_price_weights.restype = C.c_int
+    lib.orc_geo_price_weights_tol.argtypes = [_f64p, C.c_double, C.c_double, _f64p, _f64p, C.c_int64, C.c_uint32,
+                                              _f64p, C.c_double, _f64p]
+    lib.orc_geo_price_weights_tol.restype = C.c_int
     lib.orc_geo_asian_price.argtypes = [_f64p, C.c_double, C.c_double, _f64p, C.c_uint32, _u32p, C.c_uint32,
                                         C.c_uint32, _f64p, _f64p]
     lib.orc_geo_asian_price.restype = C.c_int
@@ -236,13 +239,15 @@ def geo_cf(params, r: float, t, v, s: complex) -> complex:
     return complex(out[0], out[1])
 
 
-def geo_price_weights(params, S0: float, r: float, t, w, strikes) -> np.ndarray:
-    """E[(W S0 exp(sum_j (w_j/W) ln(X_{t_j}/S0)) - K)^+], W = sum_j w_j, undiscounted."""
+def geo_price_weights(params, S0: float, r: float, t, w, strikes, tol: float = 1e-15) -> np.ndarray:
+    """E[(W S0 exp(sum_j (w_j/W) ln(X_{t_j}/S0)) - K)^+], W = sum_j w_j, undiscounted; the Fourier
+    sum stops at two consecutive panels whose |phi| is below tol u0 (NaN if it never does)."""
     t = np.ascontiguousarray(t, np.float64)
     w = np.ascontiguousarray(w, np.float64)
     K = np.ascontiguousarray(np.atleast_1d(strikes), np.float64)
     out = np.zeros(len(K), np.float64)
-    rc = load().orc_geo_price_weights(np.asarray(params, np.float64), S0, r, t, w, len(t) - 1, len(K), K, out)
+    rc = load().orc_geo_price_weights_tol(np.asarray(params, np.float64), S0, r, t, w, len(t) - 1, len(K), K,
+                                          float(tol), out)
     if rc not in (0, -4):
         raise ValueError(f"orc_geo_price_weights failed: {rc}")
     return out

@@ -329,12 +329,14 @@ def test_fp32_over_a_wide_parameter_box(ctx):
     1e-4-1, sigma 1e-3-3, rho in [-1, 1] including +-1, v0 0-1, T 0.01-30 years on 1-500 steps), with
     and without the martingale correction, Asian, European and control-variate blocks: the Asian and
     European prices are finite in both precisions, FP32 is within 1e-4 relative plus 1e-5 S0 of
-    FP64 on the same draws, and the control-variate block is finite in both. The control variate
-    is requested where its exact price exists (|rho| < 1); at rho = +-1 such a request is refused."""
+    FP64 on the same draws, and the control-variate block is finite in both. A control-variate
+    request whose exact price does not exist (rho = +-1) or does not converge (v0 = 0 with a Feller
+    ratio near 0: the variance stays at 0) is refused with HEXO_ERR_INVALID, never priced NaN; such
+    requests are priced without the control variate."""
     say("gpu fp32 over a wide parameter box")
     rng = np.random.default_rng(5)
     reqs = []
-    refused = 0
+    refused = not_converged = 0
     for i in range(300):
         T = float(10 ** rng.uniform(-2, np.log10(30)))
         steps = int(max(1, round(10 ** rng.uniform(0, np.log10(500)))))
@@ -349,7 +351,18 @@ def test_fp32_over_a_wide_parameter_box(ctx):
             assert H.validate(q) == -1
             q.payoffs = H.ASIAN | H.EUROPEAN
             refused += 1
+        elif q.payoffs & H.ASIAN_GEO_CV:
+            try:
+                ctx.geo_asian_price(q)
+            except H.HexoError as e:
+                assert e.code == -1 and "did not converge" in str(e), e
+                assert q.v0 == 0.0 and 2 * q.kappa * q.theta / q.sigma ** 2 < 0.1, (q.v0, q.kappa, q.theta, q.sigma)
+                with pytest.raises(H.HexoError):
+                    ctx.price(q)
+                q.payoffs = H.ASIAN | H.EUROPEAN
+                not_converged += 1
         reqs.append(q)
+    say(f"  control variate refused: {refused} at rho = +-1, {not_converged} not converging")
     assert refused > 0
     assert all(H.validate(q) == 0 for q in reqs)
     out64 = ctx.price_batch(reqs)

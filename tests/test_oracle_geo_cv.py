@@ -180,3 +180,41 @@ def test_exact_geometric_price_is_an_arbitrage_free_call(case):
     assert np.all(C >= EG - K - tol)
     assert np.all(np.diff(C) <= tol)
     assert np.all(np.diff(C, 2) >= -tol)
+
+
+STOP_RULE_CASES = [
+    dict(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, T=1.0, steps=52),
+    dict(v0=0.04, theta=0.04, rho=-0.9, kappa=0.5, sigma=0.9, T=1.0, steps=256),
+    dict(v0=0.01, theta=0.01, rho=-0.5, kappa=0.5, sigma=1.0, T=0.1, steps=20),
+    dict(v0=0.09, theta=0.04, rho=0.4, kappa=1.5, sigma=0.4, T=2.0, steps=24),
+    NEAR_DETERMINISTIC[2],  # the other near-deterministic cases take a minute each at 1e-18
+]
+
+
+@pytest.mark.parametrize("case", STOP_RULE_CASES)
+def test_exact_price_stop_rule_pinned_by_a_stricter_tolerance(case):
+    """The Fourier sum stops at two consecutive panels whose largest |phi| is below 1e-15 u0. The
+    rule assumes |phi| keeps that low beyond them; a sum run on to 1e-18 u0 (many more panels)
+    must give the same prices to 1e-12 of the price scale E[G]."""
+    from oracle import oracle as O
+    p = [case["v0"], case["theta"], case["rho"], case["kappa"], case["sigma"]]
+    t, w = O.asian_weights([case["T"]], [1], case["steps"], 0, 1)
+    W = w.sum()
+    EG = W * 100.0 * O.geo_cf(p, 0.02, t, w / W, 1.0 + 0j).real
+    K = np.linspace(0.8, 1.2, 9) * EG
+    c15 = O.geo_price_weights(p, 100.0, 0.02, t, w, K)
+    c18 = O.geo_price_weights(p, 100.0, 0.02, t, w, K, tol=1e-18)
+    assert np.all(np.isfinite(c15)) and np.all(np.isfinite(c18))
+    np.testing.assert_allclose(c15, c18, rtol=0, atol=1e-12 * EG)
+
+
+def test_exact_price_does_not_converge_when_the_variance_stays_at_zero():
+    """v0 = 0 with a Feller ratio 2 kappa theta / sigma^2 near 0.001: the variance stays near 0,
+    L is nearly deterministic with a heavy spread of small conditional variances, and |phi| hardly
+    decays; the sum does not meet the stop rule within the panel limit and the price reads NaN
+    (the product refuses such requests)."""
+    from oracle import oracle as O
+    p = [0.0, 0.0005971753929900366, 0.27077359978142157, 0.278160725280941, 0.5949943958586498]
+    t, w = O.asian_weights([0.02357019432969018], [1], 74, 0, 1)
+    c = O.geo_price_weights(p, 100.0, 0.02, t, w, [80.0, 100.0, 120.0])
+    assert np.all(np.isnan(c))
