@@ -1,15 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark of the Heston QE Monte-Carlo pricing path on one MI355X.
 
-Workload (the metric's configuration): the strike/maturity grid -- Heston kappa=2, theta=0.04,
-sigma=0.3, rho=-0.7, v0=0.04, S0=100, r=0.02; 64 strikes in [70,130] x 8 maturities
-{0.1,...,5}; 1e6 paths per maturity shared by all strikes and by the Asian and European payoffs;
-round(52 T) steps; FP32 stepping with FP64 reduction. One step = one pricing pass over the grid.
+Headline workload (the metric's batched configuration): 512 pricing requests with uneven path
+counts, step counts, strikes and Heston parameters (kappa U[0.5,20], theta U[0.01,0.2], sigma
+U[0.1,1], rho U[-0.95,0], v0 U[0.01,0.2], K U[80,120], T U[0.1,5], round(252 T) steps, paths
+log-uniform in [1e4,1e7], Asian and European from the same paths, r = 0.02) in one persistent,
+graph-captured launch; FP32 stepping with FP64 reduction. One step = one replay of the batch.
 
 value = nominal path-steps/s (sum of n_paths * steps) of the FP32 batched path, inputs resident.
-Also reported: the plain FP64 kernel on the same workload and draws, the FP32-vs-FP64 price
-error, the path kernel's FP32 roofline fraction (algorithmic flops from exact branch counts), the
-batched uneven-request workload, serving latency, and the CPU baseline (the oracle, OpenMP).
+Also reported: the plain FP64 kernel on the same requests and draws (full paths) and every
+request's FP32-vs-FP64 error, the path kernel's FP32 roofline fraction (algorithmic flops from
+the workload's exact branch counts), the strike/maturity grid, the single request, the stiff
+precision study, variance reduction, SWIFT and calibration, the 60 s serving stream (latency,
+capacity, errors), and the CPU baseline (the oracle, OpenMP) on the same 512 requests.
 """
 from __future__ import annotations
 
@@ -42,17 +45,30 @@ F_FIN = 6        # per path and expiry: X_T interpolation and the Asian average
 F_PAYOFF = 1     # per path and output (FP32 subtraction; the sums are FP64)
 
 
-def pmc_traffic(paths: int, tuning: dict):
+def pmc_traffic(workload: str, paths: int, tuning: dict):
     """HBM bytes per launch of the path kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary of this workload (scaled by the copy-kernel calibration), or None when this run's
     workload or launch shape is not the one profiled."""
-    f = os.path.join(ROOT, "profiles", "k_path_fp32_c2_grid_traffic.json")
+    f = os.path.join(ROOT, "profiles", f"k_path_fp32_{workload}_traffic.json")
     if not os.path.exists(f):
         return None
     d = json.load(open(f))
-    if d["paths_per_maturity"] != paths or d["paths_per_thread"] != tuning["paths_per_thread"]:
+    if d.get("paths") != paths or d["paths_per_thread"] != tuning["paths_per_thread"]:
         return None
     return float(d["hbm_bytes_per_launch"])
+
+
+def check_counts(counts, steps_run):
+    """Every simulated step takes one of the two branches and draws 2 (quadratic) or 1 (exponential)
+    normals: counts that break this are not counts, and the flop figure built on them is void."""
+    if (int(counts[0] + counts[1]) != steps_run
+            or int(counts[2] + counts[3]) != 2 * int(counts[0]) + int(counts[1])):
+        raise SystemExit(f"branch counts {counts.tolist()} disagree with the {steps_run} simulated path-steps")
+
+
+def branch_dict(counts):
+    return {"quadratic_steps": int(counts[0]), "exponential_steps": int(counts[1]),
+            "central_normals": int(counts[2]), "tail_normals": int(counts[3])}
 
 
 def log(*a):
@@ -364,6 +380,149 @@ def price_errors(ctx, reqs, p32, s32, p64, report_path=None):
             "per_maturity": summary}
 
 
+def fp32_err(p32, p64):
+    """|p32 - p64| in units of 1e-4 max(p64, 0.01): the error target on identical draws is <= 1."""
+    return np.abs(p32 - p64) / (1e-4 * np.maximum(np.abs(p64), 0.01))
+
+
+def c3_headline(ctx, args, reqs3):
+    """The batched uneven requests (C3): FP32 throughput over graph replays of the 512-request batch,
+    the path kernel's roofline from this workload's own branch counts and HIP-event kernel time,
+    the plain FP64 kernel on the same requests and draws at full paths (the speed-up's
+    denominator), and every request's FP32 error against FP64."""
+    import hestonexotics_amd as H
+    from hestonexotics_amd import workloads as W
+    nominal = W.path_steps(reqs3)
+    b = ctx.batch(reqs3)
+    simulated = b.stats()["path_steps"]
+    counts = b.branch_counts()
+    check_counts(counts, simulated)
+    flops = algorithmic_flops(reqs3, counts)
+    log("C3 branch counts (quad, exp, central, tail):", counts.tolist(), "flops/pass %.4e" % flops)
+    eager_k = max(2, min(args.warmup, 3))
+    wall_e, kms = time_batch(ctx, b, eager_k, 1)
+    p32, s32 = b.results()
+    graph_s = time_graph(b, args.steps, args.warmup)
+    value = nominal / graph_s
+    achieved = flops / (kms * 1e-3) / 1e12
+    log(f"FP32 C3: {1e3 * graph_s:.1f} ms/pass in graph replays (path kernel {kms:.1f} ms), {value:.4e} "
+        f"path-steps/s, {achieved:.2f} TFLOP/s")
+    b.close()
+    # the plain FP64 kernel, same requests and draws (full paths unless --c3-fp64-scale says otherwise)
+    scale = args.c3_fp64_scale
+    r64 = W.c3(precision=H.FP64, path_scale=scale)
+    b64 = ctx.batch(r64)
+    w64, k64 = time_batch(ctx, b64, 1, 0)
+    p64, s64 = b64.results()
+    g64 = time_graph(b64, 1, 0)
+    value64 = W.path_steps(r64) / g64
+    b64.close()
+    if scale != 1.0:  # errors on the same sample in FP32
+        bf = ctx.batch(W.c3(precision=H.FP32, path_scale=scale))
+        bf.run_eager()
+        bf.sync()
+        p32c, _ = bf.results()
+        bf.close()
+    else:
+        p32c = p32
+    err = fp32_err(p32c, p64)
+    big = p64 > 1e-2
+    rel = np.abs(p32c[big] - p64[big]) / p64[big]
+    worst = int(np.argmax(err))
+    log(f"FP64 C3: {g64:.2f} s/pass (kernel {k64:.1f} ms), {value64:.4e} path-steps/s; FP32/FP64 "
+        f"{value / value64:.2f}x; worst FP32 vs FP64 {err.max():.3f} x 1e-4")
+    return {"value": value, "ms_per_step": 1e3 * graph_s, "nominal": nominal, "simulated": simulated,
+            "eager_ms_per_step": 1e3 * wall_e / eager_k,
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / PEAK_FP32_TFLOPS,
+                         "traffic": pmc_traffic("c3", 0, ctx.get_tuning()),
+                         "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, "
+                                         "copy-calibrated)",
+                         "kernel": "k_path_fp32", "kernel_ms": kms, "algorithmic_flops_per_launch": flops,
+                         "branch_counts": branch_dict(counts)},
+            "fp64_plain": {"value": value64, "s_per_step": g64, "kernel_ms": k64, "path_scale": scale,
+                           "fp32_speedup": value / value64,
+                           # the same algorithmic work as the FP32 kernel's, against the FP64 vector peak
+                           "achieved_tflops_same_work": flops * scale / (k64 * 1e-3) / 1e12},
+            "fp32_vs_fp64": {"outputs": int(len(p64)), "worst_in_units_of_1e-4_max_price_0.01": float(err.max()),
+                             "max_rel_err_prices_above_0.01": float(rel.max()),
+                             "mean_rel_err_prices_above_0.01": float(rel.mean()),
+                             "worst_output": worst, "all_within_1e-4": bool(err.max() <= 1.0)}}
+
+
+def c1_leg(ctx):
+    """The plain single request as configured (C1: 1e5 paths, 252 steps, seed 42, FP64, r = 0.02):
+    the GPU's FP64 and FP32 prices and standard errors, the synchronous entry point's latency, and
+    the oracle's CPU FP64 price (same draws) with its time and thread count."""
+    import hestonexotics_amd as H
+    from hestonexotics_amd import workloads as W
+    from oracle import oracle as O
+    r = W.c1()
+    p64, s64 = ctx.price(r)
+    lat = {}
+    for name, q in (("fp64", r), ("fp32", W.c1(precision=H.FP32))):
+        ctx.price(q)
+        ts = []
+        for _ in range(20):
+            t0 = time.perf_counter()
+            pq, sq = ctx.price(q)
+            ts.append(time.perf_counter() - t0)
+        lat[name] = {"price": float(pq[0]), "std_err": float(sq[0]),
+                     "latency_ms_median": 1e3 * float(np.median(ts)), "latency_ms_min": 1e3 * float(np.min(ts))}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    o = O.price(r.params, r.S0, r.expiries, r.n_strikes(), r.flat_strikes(), steps=r.steps, n_paths=r.n_paths,
+                seed=r.seed, r=r.r, flags=r.flags, payoffs=r.payoffs, n_threads=threads, kind="fast")
+    return {"workload": "C1 one Asian call, S0=K=100, r=0.02, T=1, kappa 2, theta 0.04, sigma 0.3, rho -0.7, "
+                        "v0 0.04, 252 steps, 1e5 paths, seed 42",
+            "gpu_fp64": lat["fp64"], "gpu_fp32": lat["fp32"],
+            "gpu_fp64_vs_cpu_rel": float(abs(p64[0] - o["prices"][0]) / o["prices"][0]),
+            "cpu_fp64": {"price": float(o["prices"][0]), "std_err": float(o["std_errs"][0]),
+                         "seconds": float(o["seconds"]), "cores": threads, "kind": "port",
+                         "path_steps_per_s": r.n_paths * r.steps / float(o["seconds"])}}
+
+
+def serving_leg(ctx, seconds: float):
+    """The serving stream (C4: Poisson arrivals at 200 req/s, each request drawn as in C3, Asian or
+    European): latency percentiles, served throughput, the GPU's capacity on the same requests (all
+    of them priced in one batch), and the FP32 error of every 20th request against FP64."""
+    import hestonexotics_amd as H
+    from hestonexotics_amd import workloads as W
+    reqs4, arr = W.c4(duration_s=seconds, precision=H.FP32)
+    ctx.server_start(streams=4)
+    t0 = time.perf_counter()
+    rc, p4, s4, infos = ctx.run_trace(reqs4, arr)
+    wall4 = time.perf_counter() - t0
+    st = ctx.server_stats()
+    ctx.server_stop()
+    lat = np.array([i["done_s"] - i["arrival_s"] for i in infos])
+    ok = np.array([i["status"] == 0 for i in infos])
+    work = W.path_steps(reqs4)
+    # capacity: the same requests in one batch
+    bc = ctx.batch(reqs4)
+    bc.run()
+    bc.sync()
+    t1 = time.perf_counter()
+    bc.run()
+    bc.sync()
+    cap_s = time.perf_counter() - t1
+    bc.close()
+    # FP32 vs FP64 on identical draws, every 20th request
+    sample = list(range(0, len(reqs4), 20))
+    out64 = ctx.price_batch([dataclasses.replace(reqs4[i], precision=H.FP64, _keep=[]) for i in sample])
+    offs = np.cumsum([0] + [r.n_outputs for r in reqs4])
+    errs = [float(fp32_err(p4[offs[i]:offs[i + 1]], q64).max()) for i, (q64, _) in zip(sample, out64)]
+    return {"rc": rc, "requests": len(reqs4), "ok": int(ok.sum()), "rate_req_s": 200.0, "duration_s": seconds,
+            "wall_s": wall4, "offered_path_steps_per_s": work / seconds,
+            "served_path_steps_per_s": work / wall4,
+            "capacity_path_steps_per_s": work / cap_s,
+            "capacity_note": "the stream's requests priced in one batch: the GPU's rate on this mix",
+            "p50_latency_ms": float(np.percentile(lat[ok], 50) * 1e3) if ok.any() else None,
+            "p99_latency_ms": float(np.percentile(lat[ok], 99) * 1e3) if ok.any() else None,
+            "batches": st["batches"], "max_queue_depth": st["max_queue_depth"],
+            "fp32_vs_fp64": {"requests_checked": len(sample), "worst_in_units_of_1e-4_max_price_0.01": max(errs),
+                             "all_within_1e-4": bool(max(errs) <= 1.0)}}
+
+
 def time_batch(ctx, batch, steps: int, warmup: int):
     for _ in range(warmup):
         batch.run_eager()
@@ -403,15 +562,16 @@ def main():
     ap.add_argument("--price-report", default=None,
                     help="CSV file for the per-price errors of the grid workload (one row per price)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-c3", action="store_true")
-    ap.add_argument("--c3-fp64-scale", type=float, default=0.05)
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--c3-fp64-scale", type=float, default=1.0,
+                    help="path fraction of the plain FP64 kernel's run of the 512 requests (1: full paths)")
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c5-paths", type=int, default=2 ** 27, help="paths of the stiff precision-study request")
     ap.add_argument("--no-martingale", action="store_true")
     ap.add_argument("--no-cv", action="store_true")
     ap.add_argument("--no-swift", action="store_true")
     ap.add_argument("--swift-params", type=int, default=64, help="parameter sets of the SWIFT calibration batch")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape instead of running the autotuner on the workload")
     ap.add_argument("--profile", action="store_true",
@@ -487,48 +647,15 @@ def main():
                     log(f"sweep ppt={ppt} block={blk} waves_per_simd={wps}: path kernel {kms:.3f} ms")
         ctx.close()
         return
-    reqs32 = W.c2(n_paths=args.paths, precision=H.FP32)
-    nominal = W.path_steps(reqs32)
+    # ---- the headline: 512 uneven requests in one persistent launch (C3) ----
+    reqs3 = W.c3(precision=H.FP32)
     if not args.no_autotune:
         # the project's autotuner (hexo_autotune) picks paths per thread, block size and occupancy
-        # cap on the workload itself and leaves the winner set on the context
-        log("autotune:", ctx.autotune(W.c2(n_paths=args.paths, precision=H.FP32), reps=3))
+        # cap on the workload itself (a 5% path sample of it) and leaves the winner set on the context
+        log("autotune:", ctx.autotune(W.c3(precision=H.FP32, path_scale=0.05), reps=2))
     tuning = ctx.get_tuning()
-
-    b32 = ctx.batch(reqs32)
-    counts = b32.branch_counts()
-    # every simulated step takes one of the two branches and draws 2 (quadratic) or 1 (exponential)
-    # normals: counts that break this are not counts, and the flop figure built on them is void
-    steps_run = b32.stats()["path_steps"]
-    if (int(counts[0] + counts[1]) != steps_run
-            or int(counts[2] + counts[3]) != 2 * int(counts[0]) + int(counts[1])):
-        raise SystemExit(f"branch counts {counts.tolist()} disagree with the {steps_run} simulated path-steps")
-    flops = algorithmic_flops(reqs32, counts)
-    log("branch counts (quad, exp, central, tail):", counts.tolist(), "flops/pass %.4e" % flops)
-    wall32, kms32 = time_batch(ctx, b32, args.steps, args.warmup)
-    p32, s32 = b32.results()
-    graph_s = time_graph(b32, args.steps, args.warmup)
-    # the timed passes are graph replays (the batched path as the product runs it); the eager passes
-    # above, synchronised one by one, give the kernel's HIP-event time and the per-pass latency
-    ms_per_step = 1e3 * graph_s
-    value = nominal / graph_s
-    achieved = flops / (kms32 * 1e-3) / 1e12
-    log(f"FP32 C2: {ms_per_step:.3f} ms/pass in graph replays (path kernel {kms32:.3f} ms), {value:.4e} "
-        f"path-steps/s, {achieved:.2f} TFLOP/s, eager {1e3 * wall32 / args.steps:.3f} ms/pass")
-
-    # plain FP64 kernel on the same requests and draws
-    reqs64 = W.c2(n_paths=args.paths, precision=H.FP64)
-    b64 = ctx.batch(reqs64)
-    steps64 = max(2, args.steps // 10)
-    wall64, kms64 = time_batch(ctx, b64, steps64, 1)
-    p64, s64 = b64.results()
-    graph64_s = time_graph(b64, steps64)
-    value64 = nominal / graph64_s  # timed as the FP32 passes are
-    big = p64 > 1e-2
-    rel = np.abs(p32[big] - p64[big]) / p64[big]
-    se_units = np.abs(p32 - p64) / np.maximum(s64, 1e-300)
-    log(f"FP64 C2: {1e3 * graph64_s:.3f} ms/pass (kernel {kms64:.3f} ms), {value64:.4e} path-steps/s; "
-        f"FP32 vs FP64 max rel {rel.max():.3e} (prices > 0.01), max |diff|/SE {se_units.max():.3e}")
+    head = c3_headline(ctx, args, reqs3)
+    value, ms_per_step = head["value"], head["ms_per_step"]
 
     out = {
         "metric": "Heston path-steps/s FP32 vs FP64 at <=1e-4 rel price error; p50/p99 latency",
@@ -542,59 +669,74 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (FP64 reduction)",
-        "data": "synthetic (seeded Heston strike/maturity grid)",
-        "config": {"workload": "C2 strike/maturity grid: 64 strikes x 8 maturities, 1e6 paths per maturity, "
-                               "52 steps/yr, Asian+European from shared paths",
-                   "paths_per_maturity": args.paths, "nominal_path_steps_per_step": nominal,
-                   "simulated_path_steps_per_step": b32.stats()["path_steps"], "tuning": tuning,
-                   "device": info},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pmc_traffic(args.paths, tuning),
-                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)",
-                     "kernel": "k_path_fp32", "kernel_ms": kms32, "algorithmic_flops_per_launch": flops,
-                     "branch_counts": {"quadratic_steps": int(counts[0]), "exponential_steps": int(counts[1]),
-                                       "central_normals": int(counts[2]), "tail_normals": int(counts[3])}},
-        "fp64_plain": {"value": value64, "ms_per_step": 1e3 * graph64_s, "kernel_ms": kms64,
-                       "fp32_speedup": value / value64,
-                       # the same algorithmic work W as the FP32 kernel's, against the FP64 vector peak
-                       "achieved_tflops_same_work": flops / (kms64 * 1e-3) / 1e12,
-                       "frac_of_fp64_peak": flops / (kms64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
-        "fp32_vs_fp64": {"max_rel_err": float(rel.max()), "mean_rel_err": float(rel.mean()),
-                         "max_abs_err_in_se": float(se_units.max()), "n_prices": int(big.sum())},
-        "graph_replay_ms_per_step": graph_s * 1e3,
-        "eager_ms_per_step": 1e3 * wall32 / args.steps,
+        "data": "synthetic (seeded Heston parameter draws, 512 requests)",
+        "config": {"workload": "C3 batched uneven requests, one persistent graph-captured launch: 512 requests, "
+                               "kappa U[0.5,20], theta U[0.01,0.2], sigma U[0.1,1], rho U[-0.95,0], v0 U[0.01,0.2], "
+                               "K U[80,120], T U[0.1,5], round(252 T) steps, paths log-uniform [1e4,1e7], "
+                               "Asian+European from shared paths, seed per request",
+                   "requests": len(reqs3), "nominal_path_steps_per_step": head["nominal"],
+                   "simulated_path_steps_per_step": head["simulated"], "tuning": tuning, "device": info},
+        "roofline": head["roofline"],
+        "fp64_plain": head["fp64_plain"],
+        "fp32_vs_fp64": head["fp32_vs_fp64"],
+        "graph_replay_ms_per_step": ms_per_step,
+        "eager_ms_per_step": head["eager_ms_per_step"],
     }
 
+    # ---- the strike/maturity grid (C2) ----
+    reqs32 = W.c2(n_paths=args.paths, precision=H.FP32)
+    nominal = W.path_steps(reqs32)
+    b32 = ctx.batch(reqs32)
+    counts = b32.branch_counts()
+    check_counts(counts, b32.stats()["path_steps"])
+    flops = algorithmic_flops(reqs32, counts)
+    wall32, kms32 = time_batch(ctx, b32, args.steps, args.warmup)
+    p32, s32 = b32.results()
+    graph_s = time_graph(b32, args.steps, args.warmup)
+    c2_value = nominal / graph_s
+    achieved = flops / (kms32 * 1e-3) / 1e12
+    log(f"FP32 C2: {1e3 * graph_s:.3f} ms/pass in graph replays (path kernel {kms32:.3f} ms), {c2_value:.4e} "
+        f"path-steps/s, {achieved:.2f} TFLOP/s, eager {1e3 * wall32 / args.steps:.3f} ms/pass")
+    reqs64 = W.c2(n_paths=args.paths, precision=H.FP64)
+    b64 = ctx.batch(reqs64)
+    steps64 = max(2, args.steps // 10)
+    wall64, kms64 = time_batch(ctx, b64, steps64, 1)
+    p64, s64 = b64.results()
+    graph64_s = time_graph(b64, steps64)
+    value64 = nominal / graph64_s  # timed as the FP32 passes are
+    big = p64 > 1e-2
+    rel = np.abs(p32[big] - p64[big]) / p64[big]
+    se_units = np.abs(p32 - p64) / np.maximum(s64, 1e-300)
+    log(f"FP64 C2: {1e3 * graph64_s:.3f} ms/pass (kernel {kms64:.3f} ms), {value64:.4e} path-steps/s; "
+        f"FP32 vs FP64 max rel {rel.max():.3e} (prices > 0.01), max |diff|/SE {se_units.max():.3e}")
+    out["c2_grid"] = {
+        "workload": "C2 strike/maturity grid: 64 strikes x 8 maturities, 1e6 paths per maturity, 52 steps/yr, "
+                    "Asian+European from shared paths",
+        "value": c2_value, "ms_per_step": 1e3 * graph_s, "nominal_path_steps_per_step": nominal,
+        "simulated_path_steps_per_step": b32.stats()["path_steps"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": pmc_traffic("c2", args.paths, tuning),
+                     "kernel": "k_path_fp32", "kernel_ms": kms32, "algorithmic_flops_per_launch": flops,
+                     "branch_counts": branch_dict(counts)},
+        "fp64_plain": {"value": value64, "ms_per_step": 1e3 * graph64_s, "kernel_ms": kms64,
+                       "fp32_speedup": c2_value / value64},
+        "fp32_vs_fp64": {"max_rel_err": float(rel.max()), "mean_rel_err": float(rel.mean()),
+                         "max_abs_err_in_se": float(se_units.max()), "n_prices": int(big.sum())},
+    }
+    kms32_c2 = kms32
+
+    # ---- the plain single request (C1): FP64 GPU price, the oracle's CPU price and time ----
+    if not args.no_c1:
+        out["c1"] = c1_leg(ctx)
+        log("C1:", out["c1"])
+
     if not args.no_martingale:
-        out["martingale"] = martingale_leg(ctx, args, value, kms32)
+        out["martingale"] = martingale_leg(ctx, args, c2_value, kms32_c2)
         log("martingale:", out["martingale"])
 
     if not args.no_cv:
-        out["control_variate"] = control_variate_leg(ctx, args, value, kms32)
+        out["control_variate"] = control_variate_leg(ctx, args, c2_value, kms32_c2)
         log("control variate:", out["control_variate"])
-
-    if not args.no_c3:
-        c3 = W.c3(precision=H.FP32)
-        b3 = ctx.batch(c3)
-        n3 = W.path_steps(c3)
-        w3, k3 = time_batch(ctx, b3, 1, 1)
-        p3, _ = b3.results()
-        c3d = W.c3(precision=H.FP64, path_scale=args.c3_fp64_scale)
-        c3f = W.c3(precision=H.FP32, path_scale=args.c3_fp64_scale)
-        b3d = ctx.batch(c3d)
-        w3d, k3d = time_batch(ctx, b3d, 1, 0)
-        pd, sd = b3d.results()
-        b3f = ctx.batch(c3f)
-        b3f.run_eager()
-        b3f.sync()
-        pf, _ = b3f.results()
-        bigd = pd > 1e-2
-        rel3 = np.abs(pf[bigd] - pd[bigd]) / pd[bigd]
-        v3, v3d = n3 / w3, W.path_steps(c3d) / w3d
-        out["c3_batched"] = {"requests": len(c3), "nominal_path_steps": n3, "fp32_value": v3, "fp32_s": w3,
-                             "fp32_kernel_ms": k3, "fp64_value": v3d, "fp64_path_scale": args.c3_fp64_scale,
-                             "fp32_over_fp64": v3 / v3d, "fp32_vs_fp64_max_rel_err": float(rel3.max())}
-        log("C3:", out["c3_batched"])
 
     if not args.no_c5:
         # precision study on the stiff Asian call: FP32 stepping with FP64 or compensated-FP32
@@ -625,29 +767,14 @@ def main():
         log("calibration:", out["calibration"])
 
     if args.serve_seconds > 0:
-        reqs4, arr = W.c4(duration_s=args.serve_seconds, precision=H.FP32)
-        ctx.server_start(streams=4)
-        t0 = time.perf_counter()
-        rc, p4, s4, infos = ctx.run_trace(reqs4, arr)
-        wall4 = time.perf_counter() - t0
-        st = ctx.server_stats()
-        ctx.server_stop()
-        lat = np.array([i["done_s"] - i["arrival_s"] for i in infos])
-        ok = np.array([i["status"] == 0 for i in infos])
-        out["serving"] = {"rc": rc, "requests": len(reqs4), "ok": int(ok.sum()), "rate_req_s": 200.0,
-                          "duration_s": args.serve_seconds, "wall_s": wall4,
-                          "offered_path_steps_per_s": W.path_steps(reqs4) / args.serve_seconds,
-                          "achieved_path_steps_per_s": W.path_steps(reqs4) / wall4,
-                          "p50_latency_ms": float(np.percentile(lat[ok], 50) * 1e3) if ok.any() else None,
-                          "p99_latency_ms": float(np.percentile(lat[ok], 99) * 1e3) if ok.any() else None,
-                          "batches": st["batches"], "max_queue_depth": st["max_queue_depth"]}
+        out["serving"] = serving_leg(ctx, args.serve_seconds)
         log("serving:", out["serving"])
 
     if not args.no_price_errors:
         out["price_errors"] = price_errors(ctx, reqs32, p32, s32, p64, args.price_report)
         log("price errors:", {k: v for k, v in out["price_errors"].items() if k != "per_maturity"})
     if not args.no_cpu:
-        cb = cpu_baseline(W.c2(n_paths=args.paths, precision=H.FP64), args.cpu_seconds)
+        cb = cpu_baseline(W.c3(precision=H.FP64), args.cpu_seconds)
         out["cpu_baseline"] = cb
         log("cpu baseline:", cb)
     for bt in (b32, b64):

@@ -136,6 +136,45 @@ __device__ __forceinline__ float hornerf(const float (&c)[N], float r) {
     return acc;
 }
 
+// a * K + b (v_fmamk_f32) and a * b + K (v_fmaak_f32) with K a 32-bit literal and the other operands
+// in vector registers: the VOP2 forms that encode a literal. A VOP3 v_fma_f32 cannot, so for a
+// constant operand the compiler reads a scalar register instead, and on gfx950 an FP32 instruction
+// with a scalar-register operand issues at about half the rate of one without (measured). The
+// result is the same fused multiply-add, bit for bit.
+template <uint32_t K>
+__device__ __forceinline__ float fmamk_lit(float a, float b) {
+    float d;
+    asm("v_fmamk_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "i"(K), "v"(b));
+    return d;
+}
+template <uint32_t K>
+__device__ __forceinline__ float fmaak_lit(float a, float b) {
+    float d;
+    asm("v_fmaak_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "i"(K));
+    return d;
+}
+#define HEXO_F32_BITS(x) __builtin_bit_cast(uint32_t, (float)(x))
+
+// A value the compiler must keep in a vector register (it cannot prove the result wave-uniform and
+// fold it back into a scalar operand).
+__device__ __forceinline__ float in_vgpr(float x) {
+    float d;
+    asm("v_mov_b32 %0, %1" : "=v"(d) : "v"(x));
+    return d;
+}
+
+// The PPND7 central rational's constants that its literal-form Horner steps cannot encode: the
+// region's 0.180625 (in r = 0.180625 - q^2, a VOP3 fused multiply-add) and the second coefficients
+// of A and B (the first step, r a3 + a2, takes a3 as the literal). Made once per thread.
+struct Ppnd7Vregs {
+    float c, a2, b2;  // central region
+    float c2, d1;     // tail region: the second coefficients of C and D
+};
+__device__ __forceinline__ Ppnd7Vregs ppnd7_vregs() {
+    return Ppnd7Vregs{in_vgpr(0.180625f), in_vgpr(Ppnd7::a[2]), in_vgpr(Ppnd7::b[2]), in_vgpr(Ppnd7::c[2]),
+                      in_vgpr(Ppnd7::d[1])};
+}
+
 // N(0,1) in FP32 from a 32-bit word through AS241 PPND7 (the FP32 kernel's inverse normal).
 // q = (k - 2^31 + 1/2) 2^-32 and the tail argument min(u, 1-u) = (min(k, ~k) + 1/2) 2^-32 are
 // formed from the integer, so both tails keep full relative precision (no 1 - u cancellation).
@@ -143,30 +182,44 @@ __device__ __forceinline__ float central_q_f32(uint32_t k) {
     const int32_t sk = (int32_t)(k ^ 0x80000000u);
     return __builtin_fmaf((float)sk, 0x1p-32f, 0x1p-33f);
 }
-// Central region: q A(r)/B(r) with r = 0.180625 - q^2, meaningful for |q| <= 0.425.
-__device__ __forceinline__ float normal_central_rational_f32(float q) {
-    const float rc = __builtin_fmaf(-q, q, 0.180625f);
-    return (q * hornerf(Ppnd7::a, rc)) * __builtin_amdgcn_rcpf(hornerf(Ppnd7::b, rc));
+// Central region: q A(r)/B(r) with r = 0.180625 - q^2, meaningful for |q| <= 0.425. The Horner
+// steps of A and B, ((a3 r + a2) r + a1) r + a0 and likewise B, are the same fused multiply-adds as
+// hornerf's, in the literal forms above.
+__device__ __forceinline__ float normal_central_rational_f32(float q, const Ppnd7Vregs& k) {
+    const float rc = __builtin_fmaf(-q, q, k.c);
+    float A = fmamk_lit<HEXO_F32_BITS(Ppnd7::a[3])>(rc, k.a2);
+    A = fmaak_lit<HEXO_F32_BITS(Ppnd7::a[1])>(A, rc);
+    A = fmaak_lit<HEXO_F32_BITS(Ppnd7::a[0])>(A, rc);
+    float B = fmamk_lit<HEXO_F32_BITS(Ppnd7::b[3])>(rc, k.b2);
+    B = fmaak_lit<HEXO_F32_BITS(Ppnd7::b[1])>(B, rc);
+    B = fmaak_lit<HEXO_F32_BITS(Ppnd7::b[0])>(B, rc);
+    return (q * A) * __builtin_amdgcn_rcpf(B);
 }
 // The central value of word k; `central` = |q| <= 0.425 (the value is meaningless otherwise).
 __device__ __forceinline__ float normal_central_f32(uint32_t k, bool& central) {
     const float q = central_q_f32(k);
     central = __builtin_fabsf(q) <= 0.425f;
-    return normal_central_rational_f32(q);
+    return normal_central_rational_f32(q, ppnd7_vregs());
 }
-// Tail region (|q| > 0.425): r = sqrt(-ln(min(u, 1-u))) - 1.6, C(r)/D(r) with the sign of q.
-__device__ __forceinline__ float normal_tail_f32(uint32_t k) {
+// Tail region (|q| > 0.425): r = sqrt(-ln(min(u, 1-u))) - 1.6, C(r)/D(r) with the sign of q; the
+// Horner steps of C and D in the literal forms, as the central rational's.
+__device__ __forceinline__ float normal_tail_f32(uint32_t k, const Ppnd7Vregs& pk) {
     const int32_t sk = (int32_t)(k ^ 0x80000000u);
     const uint32_t mn = k ^ ~(uint32_t)(sk >> 31);
     const float rr = __builtin_fmaf((float)mn, 0x1p-32f, 0x1p-33f);
     const float rt = __builtin_amdgcn_sqrtf(-0.693147180559945309f * __builtin_amdgcn_logf(rr)) - 1.6f;
-    const float z = hornerf(Ppnd7::c, rt) * __builtin_amdgcn_rcpf(hornerf(Ppnd7::d, rt));
+    float C = fmamk_lit<HEXO_F32_BITS(Ppnd7::c[3])>(rt, pk.c2);
+    C = fmaak_lit<HEXO_F32_BITS(Ppnd7::c[1])>(C, rt);
+    C = fmaak_lit<HEXO_F32_BITS(Ppnd7::c[0])>(C, rt);
+    float D = fmamk_lit<HEXO_F32_BITS(Ppnd7::d[2])>(rt, pk.d1);
+    D = fmaak_lit<HEXO_F32_BITS(Ppnd7::d[0])>(D, rt);
+    const float z = C * __builtin_amdgcn_rcpf(D);
     return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, z) ^ ((uint32_t)sk & 0x80000000u));
 }
 __device__ __forceinline__ float normal_f32(uint32_t k) {
     bool central;
     const float zc = normal_central_f32(k, central);
-    return central ? zc : normal_tail_f32(k);
+    return central ? zc : normal_tail_f32(k, ppnd7_vregs());
 }
 
 }  // namespace hexo

@@ -180,18 +180,24 @@ __device__ __forceinline__ uint32_t payoff_kind(uint32_t payoffs, uint32_t pay) 
 // Payoff of one path for strike K and payoff kind. Path values: the FP32 kernel's float2 (A, X_T)
 // or float4 (A, X_T, G, -) in units of S0, paths beyond n_paths holding -inf; the FP64 kernel's
 // double4 (A, X_T, G, -) with an explicit validity flag. The control-variate payoff is the
-// arithmetic minus the geometric call payoff.
+// arithmetic minus the geometric call payoff. max(d, 0) keeps a NaN (fmax would return 0): a path
+// whose price left the floating-point range (inf - inf in the finalisation) makes the price NaN
+// instead of silently 0.
+template <typename T>
+__device__ __forceinline__ T call_payoff(T d) {
+    return d > T(0) ? d : (d == d ? T(0) : d);
+}
 __device__ __forceinline__ float payoff_of(float2 v, uint32_t kind, float K, bool) {
-    return __builtin_fmaxf((kind == kEuropean ? v.y : v.x) - K, 0.0f);
+    return call_payoff((kind == kEuropean ? v.y : v.x) - K);
 }
 __device__ __forceinline__ float payoff_of(float4 v, uint32_t kind, float K, bool) {
-    const float pg = __builtin_fmaxf(v.z - K, 0.0f);
-    const float pa = __builtin_fmaxf((kind == kEuropean ? v.y : v.x) - K, 0.0f);
+    const float pg = call_payoff(v.z - K);
+    const float pa = call_payoff((kind == kEuropean ? v.y : v.x) - K);
     return kind == kGeoAsian ? pg : kind == kAsianGeoCV ? pa - pg : pa;
 }
 __device__ __forceinline__ double payoff_of(double4 v, uint32_t kind, double K, bool valid) {
-    const double pg = __builtin_fmax(v.z - K, 0.0);
-    const double pa = __builtin_fmax((kind == kEuropean ? v.y : v.x) - K, 0.0);
+    const double pg = call_payoff(v.z - K);
+    const double pa = call_payoff((kind == kEuropean ? v.y : v.x) - K);
     const double pay = kind == kGeoAsian ? pg : kind == kAsianGeoCV ? pa - pg : pa;
     return valid ? pay : 0.0;
 }
@@ -304,7 +310,7 @@ struct WaveLds {
 // from the same address at a fixed offset. q must be wave-uniform.
 template <int N>
 __device__ __forceinline__ void tail_normals(const uint32_t (&w)[N], const bool (&t)[N], const uint64_t (&mk)[N],
-                                             float (&z)[N], uint32_t* q, int lane) {
+                                             float (&z)[N], uint32_t* q, int lane, const Ppnd7Vregs& pk) {
     uint32_t rank[N], base[N];
     uint32_t total = 0;  // wave-uniform
 #pragma unroll
@@ -318,7 +324,7 @@ __device__ __forceinline__ void tail_normals(const uint32_t (&w)[N], const bool 
     wave_sync_lds();
     for (uint32_t p = 0; p < total; p += kTile) {
         const uint32_t i = p + (uint32_t)lane;
-        if (i < total) q_out[i] = normal_tail_f32(q[i]);
+        if (i < total) q_out[i] = normal_tail_f32(q[i], pk);
     }
     wave_sync_lds();
 #pragma unroll
@@ -454,10 +460,10 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
 template <int N>
 __device__ __forceinline__ void queue_normals(uint32_t wv, uint32_t wx, bool need_v, int s0, float (&z)[N],
                                               uint32_t (&w)[N], bool (&t)[N], uint64_t (&mk)[N], bool& tv,
-                                              bool& tx) {
+                                              bool& tx, const Ppnd7Vregs& pk) {
     const float qv = central_q_f32(wv), qx = central_q_f32(wx);
-    z[s0] = normal_central_rational_f32(qv);
-    z[s0 + 1] = normal_central_rational_f32(qx);
+    z[s0] = normal_central_rational_f32(qv, pk);
+    z[s0 + 1] = normal_central_rational_f32(qx, pk);
     w[s0] = wv;
     w[s0 + 1] = wx;
     tv = __builtin_fabsf(qv) > 0.425f;
@@ -484,7 +490,7 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
                                              float (&P)[PPT], float (&Yp)[PPT], float (&PY)[PPT],
                                              const uint32_t (&wv)[PPT], const uint32_t (&wx)[PPT],
                                              const SegF& c, uint32_t* q, int lane, const bool (&valid)[PPT],
-                                             StepCount& cnt) {
+                                             StepCount& cnt, const Ppnd7Vregs& pk) {
     float d0[PPT], m[PPT], s2h[PPT], m2[PPT], tt[PPT], zv[PPT], zx[PPT];
     bool quad[PPT];
     uint32_t w[2 * PPT];
@@ -495,10 +501,10 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         bool tv, tx;
-        queue_normals<2 * PPT>(wv[j], wx[j], quad[j], 2 * j, z, w, t, mk, tv, tx);
+        queue_normals<2 * PPT>(wv[j], wx[j], quad[j], 2 * j, z, w, t, mk, tv, tx, pk);
         if constexpr (COUNT) count_step(cnt, valid[j], quad[j], tv, tx);
     }
-    tail_normals<2 * PPT>(w, t, mk, z, q, lane);
+    tail_normals<2 * PPT>(w, t, mk, z, q, lane, pk);
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         zv[j] = z[2 * j];
@@ -550,6 +556,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
         Yp[j] = PY[j] = accY[j] = Ys[j] = 0.0f;
         sv[j] = sx[j] = 0u;
     }
+    const Ppnd7Vregs pk = ppnd7_vregs();
     uint32_t k = 0;  // global step index; step k uses words 2k, 2k+1 of the path's stream
     const uint32_t n_segs = R.n_segs;
     for (uint32_t s = 0; s < n_segs; ++s) {
@@ -558,20 +565,20 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
         const uint32_t n = S.n_steps;
         uint32_t i = 0;
         if ((k & 1u) && n > 0) {  // second half of a block opened by the previous run
-            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, sv, sx, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, sv, sx, c, lds, lane, valid, cnt, pk);
             ++i;
             ++k;
         }
         for (; i + 2 <= n; i += 2, k += 2) {
             uint32_t w0[PPT], w1[PPT], w2[PPT], w3[PPT];
             philox_block<PPT>(k >> 1, plo, phi, key0, key1, w0, w1, w2, w3);
-            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w0, w1, c, lds, lane, valid, cnt);
-            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w2, w3, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w0, w1, c, lds, lane, valid, cnt, pk);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w2, w3, c, lds, lane, valid, cnt, pk);
         }
         if (i < n) {
             uint32_t w0[PPT], w1[PPT];
             philox_block<PPT>(k >> 1, plo, phi, key0, key1, w0, w1, sv, sx);
-            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w0, w1, c, lds, lane, valid, cnt);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w0, w1, c, lds, lane, valid, cnt, pk);
             ++k;
         }
         // Asian accumulation of the run's interior steps b..e-1:
@@ -637,7 +644,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
 }
 
 template <int PPT, bool DEBUG, bool COUNT, bool MART, bool GEO>
-__global__ __launch_bounds__(256) void k_path_fp32(PathArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_path_fp32(PathArgs a) {
     extern __shared__ uint32_t lds_u32[];
     const int lane = (int)(threadIdx.x & 63u);
     // the wave's LDS region, at a wave-uniform address
