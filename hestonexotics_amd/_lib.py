@@ -115,6 +115,7 @@ SIGNATURES = {
     "hexo_ctx_last_error": (C.c_char_p, [_P]),
     "hexo_device_info": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hexo_price": (C.c_int, [_P, _req, _d, _d]),
+    "hexo_price_noalloc": (C.c_int, [_P, _req, _d, _d]),
     "hexo_price_batch": (C.c_int, [_P, _req, C.c_uint32, _d, _d]),
     "hexo_batch_create": (C.c_int, [_P, _req, C.c_uint32, C.POINTER(_P)]),
     "hexo_batch_run": (C.c_int, [_P]),

@@ -133,6 +133,12 @@ int hexo_ctx_create(int device, hexo_ctx** out) {
         delete c;
         return HEXO_ERR_HIP;
     }
+    // hexo_price_noalloc's fixed region on this device (its address, looked up with the device
+    // current); on failure that entry point reports it and everything else works
+    if (hexo::scratch_region(device, &c->scratch, &c->scratch_bytes) != HEXO_OK) {
+        c->scratch = nullptr;
+        c->scratch_bytes = 0;
+    }
     *out = c;
     return HEXO_OK;
 }

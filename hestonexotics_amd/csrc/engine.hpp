@@ -231,4 +231,8 @@ void batch_destroy(Batch& b);
 
 double now_s();
 
+// The allocation-free entry point's fixed device region on `device` (current), looked up once per
+// device (noalloc.hip). HEXO_OK or an error code.
+int scratch_region(int device, void** ptr, size_t* bytes);
+
 }  // namespace hexo

@@ -159,6 +159,13 @@ int hexo_device_info(hexo_ctx* ctx, char* name, size_t name_len, int* n_cu, int*
  * hexo_price replaces HSimulation::price<HQEAnderson<ffloat,Policy>> (src/HSimulation.tpp:10-51):
  * prices[o], std_errs[o] for o < hexo_request_n_outputs(req). Synchronous. std_errs may be NULL. */
 int hexo_price(hexo_ctx* ctx, const hexo_request* req, double* prices, double* std_errs);
+/* No counterpart in the reference. hexo_price without any device or pinned-host allocation, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the request's device region lives in a fixed 64 MiB array that each GPU holds
+ * for the process's lifetime, its host staging in a fixed 16 MiB host array. Same results as
+ * hexo_price. A request needing more is refused with HEXO_ERR_INVALID. Calls are serialised by the
+ * library (the arrays are shared); each call overwrites the previous one's contents. */
+int hexo_price_noalloc(hexo_ctx* ctx, const hexo_request* req, double* prices, double* std_errs);
 /* Many requests in one persistent launch; outputs concatenated in request order. */
 int hexo_price_batch(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, double* prices, double* std_errs);
 
