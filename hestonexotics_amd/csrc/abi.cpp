@@ -440,7 +440,7 @@ int hexo_get_tuning(hexo_ctx* ctx, hexo_tuning* t) {
 int hexo_set_tuning(hexo_ctx* ctx, const hexo_tuning* t) {
     if (!ctx || !t) return HEXO_ERR_INVALID;
     const uint32_t p = t->paths_per_thread, b = t->block_threads;
-    if (!(p == 1 || p == 2 || p == 4)) return ctx->set_error(HEXO_ERR_INVALID, "paths_per_thread must be 1, 2 or 4");
+    if (!(p >= 1 && p <= 4)) return ctx->set_error(HEXO_ERR_INVALID, "paths_per_thread must be 1, 2, 3 or 4");
     if (!(b == 64 || b == 128 || b == 256)) return ctx->set_error(HEXO_ERR_INVALID, "block_threads must be 64, 128 or 256");
     if (t->waves_per_simd > 8) return ctx->set_error(HEXO_ERR_INVALID, "waves_per_simd must be <= 8");
     ctx->set_tuning(*t);
@@ -457,7 +457,7 @@ int hexo_autotune(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, uint32_t 
     // thread may be reading) changes once, to the winner
     hexo_tuning winner = ctx->tuning_now();
     double best_ms = 1e300;
-    const uint32_t ppts[] = {1, 2, 4};
+    const uint32_t ppts[] = {1, 2, 3, 4};
     const uint32_t blocks[] = {64, 128, 256};
     const uint32_t waves[] = {0, 2, 4};
     for (uint32_t p : ppts)

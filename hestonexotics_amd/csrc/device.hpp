@@ -116,26 +116,6 @@ __device__ __forceinline__ double normal_f64(uint32_t k) {
     return (q < 0.0) ? -z : z;
 }
 
-// ---- AS241 PPND7 coefficients: the single-precision routine of the same algorithm (Wichura,
-// Appl. Statist. 37(3), 1988), accurate to about 1 part in 10^7 -- the resolution of FP32.
-// Central region (|q| <= 0.425): q A(r)/B(r), r = 0.180625 - q^2; tail (r = sqrt(-ln min(u,1-u))
-// <= 5): C(r - 1.6)/D(r - 1.6). The far-tail pair (r > 5) is unreachable from 32-bit words:
-// min(u, 1-u) >= 2^-33 gives r <= 4.78.
-struct Ppnd7 {
-    static constexpr float a[4] = {3.3871327179e0f, 5.0434271938e1f, 1.5929113202e2f, 5.9109374720e1f};
-    static constexpr float b[4] = {1.0f, 1.7895169469e1f, 7.8757757664e1f, 6.7187563600e1f};
-    static constexpr float c[4] = {1.4234372777e0f, 2.7568153900e0f, 1.3067284816e0f, 1.7023821103e-1f};
-    static constexpr float d[3] = {1.0f, 7.3700164250e-1f, 1.2021132975e-1f};
-};
-
-template <int N>
-__device__ __forceinline__ float hornerf(const float (&c)[N], float r) {
-    float acc = c[N - 1];
-#pragma unroll
-    for (int i = N - 2; i >= 0; --i) acc = __builtin_fmaf(acc, r, c[i]);
-    return acc;
-}
-
 // a * K + b (v_fmamk_f32) and a * b + K (v_fmaak_f32) with K a 32-bit literal and the other operands
 // in vector registers: the VOP2 forms that encode a literal. A VOP3 v_fma_f32 cannot, so for a
 // constant operand the compiler reads a scalar register instead, and on gfx950 an FP32 instruction
@@ -162,64 +142,84 @@ __device__ __forceinline__ float in_vgpr(float x) {
     asm("v_mov_b32 %0, %1" : "=v"(d) : "v"(x));
     return d;
 }
+__device__ __forceinline__ uint32_t in_vgpr_u32(uint32_t x) {
+    uint32_t d;
+    asm("v_mov_b32 %0, %1" : "=v"(d) : "v"(x));
+    return d;
+}
 
-// The PPND7 central rational's constants that its literal-form Horner steps cannot encode: the
-// region's 0.180625 (in r = 0.180625 - q^2, a VOP3 fused multiply-add) and the second coefficients
-// of A and B (the first step, r a3 + a2, takes a3 as the literal). Made once per thread.
-struct Ppnd7Vregs {
-    float c, a2, b2;  // central region
-    float c2, d1;     // tail region: the second coefficients of C and D
+// ---- The FP32 kernel's inverse normal ----
+// z = Phi^-1(u) for u = (k + 1/2) 2^-32, the uniform of word k (as uniform_f64). The word is folded
+// onto the lower half, j = min(k, ~k), so that s = (j + 1/2) 2^-32 = min(u, 1 - u) comes from the
+// integer with only the conversion's rounding and both tails keep full relative precision. With
+// t = s (1 - s) and l = log2 t (in [-33, -2] for 32-bit words), |z| = (1 - 2s) h(l), where h is smooth
+// in l: Giles' variable for erfinv (M. Giles, "Approximating the erfinv function", GPU Computing
+// Gems Jade Edition, 2011: w = -ln(1 - x^2) with x = 1 - 2s, i.e. w = -(l + 2) ln 2). h is a degree-9
+// polynomial in l + 6.5 where l >= -11 (s >= 4.9e-4: all but about 1 draw in 1024) and a degree-8
+// polynomial in sqrt(-l) - 4.5306 below. The coefficients are weighted least-squares fits of
+// -ndtri(s) / (1 - 2s) by normal_f32_fit.py (this directory). Evaluated in FP32 (hardware log2 and
+// square root, fused multiply-adds), the result is within 7e-7 of Phi^-1(u) for l >= -11 and within
+// 3e-7 relative below (tests/test_oracle_normal_f32.py pins the restatement in the oracle).
+struct NormalF32 {
+    static constexpr float lc = -11.0f;  // the tail polynomial applies below
+    static constexpr float xc = 6.5f;    // central variable x = l + xc
+    static constexpr float c[10] = {2.3366379737854004f,     -0.23548607528209686f,    -0.004091143608093262f,
+                                    0.00035097255022265017f, 6.123504863353446e-05f,   3.1161430342763197e-06f,
+                                    -2.392110900473199e-07f, -4.523901608877168e-08f,  -7.092287312993051e-10f,
+                                    2.1350940460074241e-10f};
+    static constexpr float yc = 4.5305938720703125f;  // tail variable y = sqrt(-l) - yc
+    static constexpr float d[9] = {4.8360700607299805f,      1.2490957975387573f,     -0.011836512014269829f,
+                                   0.0019035495351999998f,   -0.00013401612523011863f, -0.00016207653970923275f,
+                                   0.00019050169794354588f,  -0.00012945415801368654f, 3.693158214446157e-05f};
 };
-__device__ __forceinline__ Ppnd7Vregs ppnd7_vregs() {
-    return Ppnd7Vregs{in_vgpr(0.180625f), in_vgpr(Ppnd7::a[2]), in_vgpr(Ppnd7::b[2]), in_vgpr(Ppnd7::c[2]),
-                      in_vgpr(Ppnd7::d[1])};
+
+// The constants of the evaluation that the literal-form instructions cannot encode (the second
+// operand of a first Horner step, 2^-33, the sign bit), made once per thread in vector registers.
+struct NormalVregs {
+    float c8, h33;
+    uint32_t sign;
+};
+__device__ __forceinline__ NormalVregs normal_vregs() {
+    return NormalVregs{in_vgpr(NormalF32::c[8]), in_vgpr(0x1p-33f), in_vgpr_u32(0x80000000u)};
 }
 
-// N(0,1) in FP32 from a 32-bit word through AS241 PPND7 (the FP32 kernel's inverse normal).
-// q = (k - 2^31 + 1/2) 2^-32 and the tail argument min(u, 1-u) = (min(k, ~k) + 1/2) 2^-32 are
-// formed from the integer, so both tails keep full relative precision (no 1 - u cancellation).
-__device__ __forceinline__ float central_q_f32(uint32_t k) {
-    const int32_t sk = (int32_t)(k ^ 0x80000000u);
-    return __builtin_fmaf((float)sk, 0x1p-32f, 0x1p-33f);
-}
-// Central region: q A(r)/B(r) with r = 0.180625 - q^2, meaningful for |q| <= 0.425. The Horner
-// steps of A and B, ((a3 r + a2) r + a1) r + a0 and likewise B, are the same fused multiply-adds as
-// hornerf's, in the literal forms above.
-__device__ __forceinline__ float normal_central_rational_f32(float q, const Ppnd7Vregs& k) {
-    const float rc = __builtin_fmaf(-q, q, k.c);
-    float A = fmamk_lit<HEXO_F32_BITS(Ppnd7::a[3])>(rc, k.a2);
-    A = fmaak_lit<HEXO_F32_BITS(Ppnd7::a[1])>(A, rc);
-    A = fmaak_lit<HEXO_F32_BITS(Ppnd7::a[0])>(A, rc);
-    float B = fmamk_lit<HEXO_F32_BITS(Ppnd7::b[3])>(rc, k.b2);
-    B = fmaak_lit<HEXO_F32_BITS(Ppnd7::b[1])>(B, rc);
-    B = fmaak_lit<HEXO_F32_BITS(Ppnd7::b[0])>(B, rc);
-    return (q * A) * __builtin_amdgcn_rcpf(B);
-}
-// The central value of word k; `central` = |q| <= 0.425 (the value is meaningless otherwise).
-__device__ __forceinline__ float normal_central_f32(uint32_t k, bool& central) {
-    const float q = central_q_f32(k);
-    central = __builtin_fabsf(q) <= 0.425f;
-    return normal_central_rational_f32(q, ppnd7_vregs());
-}
-// Tail region (|q| > 0.425): r = sqrt(-ln(min(u, 1-u))) - 1.6, C(r)/D(r) with the sign of q; the
-// Horner steps of C and D in the literal forms, as the central rational's.
-__device__ __forceinline__ float normal_tail_f32(uint32_t k, const Ppnd7Vregs& pk) {
-    const int32_t sk = (int32_t)(k ^ 0x80000000u);
-    const uint32_t mn = k ^ ~(uint32_t)(sk >> 31);
-    const float rr = __builtin_fmaf((float)mn, 0x1p-32f, 0x1p-33f);
-    const float rt = __builtin_amdgcn_sqrtf(-0.693147180559945309f * __builtin_amdgcn_logf(rr)) - 1.6f;
-    float C = fmamk_lit<HEXO_F32_BITS(Ppnd7::c[3])>(rt, pk.c2);
-    C = fmaak_lit<HEXO_F32_BITS(Ppnd7::c[1])>(C, rt);
-    C = fmaak_lit<HEXO_F32_BITS(Ppnd7::c[0])>(C, rt);
-    float D = fmamk_lit<HEXO_F32_BITS(Ppnd7::d[2])>(rt, pk.d1);
-    D = fmaak_lit<HEXO_F32_BITS(Ppnd7::d[0])>(D, rt);
-    const float z = C * __builtin_amdgcn_rcpf(D);
-    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, z) ^ ((uint32_t)sk & 0x80000000u));
+// The FP32 normal of word k. `tail` reports l < lc; the tail polynomial is evaluated only where
+// `need` is also set (a caller that does not use the value passes false: it then holds the central
+// polynomial's value, meaningless in the tail).
+__device__ __forceinline__ float normal_word_f32(uint32_t k, bool need, const NormalVregs& nv, bool& tail) {
+    const uint32_t j = k ^ (uint32_t)((int32_t)k >> 31);
+    const float s = fmamk_lit<HEXO_F32_BITS(0x1p-32f)>((float)j, nv.h33);
+    const float t = __builtin_fmaf(-s, s, s);
+    const float l = __builtin_amdgcn_logf(t);
+    const float x = l + NormalF32::xc;
+    float h = fmamk_lit<HEXO_F32_BITS(NormalF32::c[9])>(x, nv.c8);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[7])>(h, x);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[6])>(h, x);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[5])>(h, x);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[4])>(h, x);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[3])>(h, x);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[2])>(h, x);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[1])>(h, x);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[0])>(h, x);
+    tail = l < NormalF32::lc;
+    if (need && tail) {
+        const float y = __builtin_amdgcn_sqrtf(-l) - NormalF32::yc;
+        float g = fmamk_lit<HEXO_F32_BITS(NormalF32::d[8])>(y, in_vgpr(NormalF32::d[7]));
+        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[6])>(g, y);
+        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[5])>(g, y);
+        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[4])>(g, y);
+        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[3])>(g, y);
+        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[2])>(g, y);
+        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[1])>(g, y);
+        h = fmaak_lit<HEXO_F32_BITS(NormalF32::d[0])>(g, y);
+    }
+    const float pr = h * __builtin_fmaf(-2.0f, s, 1.0f);  // |z| = h (1 - 2s)
+    // negative for k < 2^31 (u < 1/2)
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, pr) ^ (~k & nv.sign));
 }
 __device__ __forceinline__ float normal_f32(uint32_t k) {
-    bool central;
-    const float zc = normal_central_f32(k, central);
-    return central ? zc : normal_tail_f32(k, ppnd7_vregs());
+    bool tail;
+    return normal_word_f32(k, true, normal_vregs(), tail);
 }
 
 }  // namespace hexo

@@ -344,9 +344,13 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
     }
     const hexo_tuning tun = tuning ? *tuning : ctx->tuning_now();
     // tiles per work item of each kernel class: 1 for the FP64 kernel, the FP32 kernels' paths per
-    // thread; a wave stores one partial per item and output (layout.hpp, kChunkTiles)
-    uint32_t cls_ppt[kPathClasses];
-    for (uint32_t cls = 0; cls < kPathClasses; ++cls) cls_ppt[cls] = cls == 0 ? 1u : tun.paths_per_thread;
+    // thread; a wave stores one partial per item and output (layout.hpp, kChunkTiles), or one per
+    // tile when its tiles do not divide a chunk (3 paths per thread)
+    uint32_t cls_ppt[kPathClasses], cls_store[kPathClasses];
+    for (uint32_t cls = 0; cls < kPathClasses; ++cls) {
+        cls_ppt[cls] = cls == 0 ? 1u : tun.paths_per_thread;
+        cls_store[cls] = kChunkTiles % cls_ppt[cls] == 0 ? cls_ppt[cls] : 1u;
+    }
     std::vector<ReqPlan> plans(n);
     uint64_t n_seg = 0, n_fin = 0, n_opt = 0, n_partials = 0, n_groups = 0, n_out = 0;
     uint64_t n_geo_items = 0, n_geo_pts = 0, n_geo_waves = 0;
@@ -372,7 +376,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         n_seg += p.segs.size();
         n_fin += p.n_fins;
         n_opt += p.n_opts;
-        n_partials += (uint64_t)((p.n_tiles + cls_ppt[p.cls] - 1) / cls_ppt[p.cls]) * p.n_out;
+        n_partials += (uint64_t)((p.n_tiles + cls_store[p.cls] - 1) / cls_store[p.cls]) * p.n_out;
         n_groups += (uint64_t)p.n_groups * p.n_out;
         n_out += p.n_out;
         n_cls[p.cls]++;
@@ -472,7 +476,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         d.n_exp = q.n_expiries;
         d.n_groups = p.n_groups;
         d.v0_f = (float)q.v0;
-        d.store_tiles = cls_ppt[p.cls];
+        d.store_tiles = cls_store[p.cls];
         // options (chain-major) with their expiry's discount factor
         uint64_t oo = 0;
         std::vector<uint32_t> chain_opt_begin(q.n_expiries);
@@ -563,7 +567,7 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
         }
         b->out_base[i] = out_i;
         opt_i += p.n_opts;
-        part_i += (uint64_t)((p.n_tiles + cls_ppt[p.cls] - 1) / cls_ppt[p.cls]) * p.n_out;
+        part_i += (uint64_t)((p.n_tiles + cls_store[p.cls] - 1) / cls_store[p.cls]) * p.n_out;
         grp_i += (uint64_t)p.n_groups * p.n_out;
         out_i += p.n_out;
         ge += (uint64_t)p.n_groups * p.n_out;

@@ -223,7 +223,7 @@ int price_all(const std::string& sym, const std::string& vol_type, long vol_n) {
 
 // Moments of 2^26 uniforms and normals drawn on the GPU (src/UnitTest.cpp:528-564 draws 2^28).
 // The reference's rng_test (src/UnitTest.cpp:528-564) at its sample size: 2^28 uniforms and 2^28
-// normals (the FP32 path's PPND7) from the Philox stream, mean and variance of each against the
+// normals (the FP32 path's inverse normal) from the Philox stream, mean and variance of each against the
 // exact moments. The reference's bound 1e-4 is kept where it is at least five standard errors at
 // this size (the uniform's mean, 5.5 SE, and variance, 22 SE); the normal's mean and variance
 // have standard errors 6.1e-5 and 8.6e-5, so they are held to five of those.

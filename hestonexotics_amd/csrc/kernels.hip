@@ -1,6 +1,6 @@
 // kernels.hip -- HIP kernels of the Heston QE Monte-Carlo path for gfx950.
 //
-// k_path_fp32  fused FP32 path kernel: Philox4x32-10 -> AS241 -> QE step -> Asian/European
+// k_path_fp32  fused FP32 path kernel: Philox4x32-10 -> inverse normal -> QE step -> Asian/European
 //              finalisation -> per-tile payoff sums. Replaces the reference's OpenMP path loop
 //              and scheme (src/HSimulation.tpp:23-49, :52-86), the option policies
 //              (src/inc/AsianContract.h:25-47, src/inc/VanillaContract.h:24-39) and the RNG ring
@@ -108,6 +108,7 @@ struct TileSums<double> {  // FP64 sums (HEXO_FP32 and the FP64 kernel)
         s += o.s;
         ss += o.ss;
     }
+    __device__ __forceinline__ void poison() { s = ss = __builtin_nan(""); }
     __device__ __forceinline__ void load(const void* partials, size_t slot) {
         const double2 v = reinterpret_cast<const double2*>(partials)[slot];
         s = v.x;
@@ -136,6 +137,7 @@ struct TileSums<float> {  // compensated FP32 sums (HEXO_FP32_COMPENSATED)
         s.merge(o.s.s, o.s.c);
         ss.merge(o.ss.s, o.ss.c);
     }
+    __device__ __forceinline__ void poison() { s = ss = CompF{__builtin_nanf(""), 0.0f}; }
     __device__ __forceinline__ void load(const void* partials, size_t slot) {
         const float4 v = reinterpret_cast<const float4*>(partials)[slot];
         s = CompF{v.x, v.y};
@@ -180,31 +182,69 @@ __device__ __forceinline__ uint32_t payoff_kind(uint32_t payoffs, uint32_t pay) 
 // Payoff of one path for strike K and payoff kind. Path values: the FP32 kernel's float2 (A, X_T)
 // or float4 (A, X_T, G, -) in units of S0, paths beyond n_paths holding -inf; the FP64 kernel's
 // double4 (A, X_T, G, -) with an explicit validity flag. The control-variate payoff is the
-// arithmetic minus the geometric call payoff. max(d, 0) keeps a NaN (fmax would return 0): a path
-// whose price left the floating-point range (inf - inf in the finalisation) makes the price NaN
-// instead of silently 0.
-template <typename T>
-__device__ __forceinline__ T call_payoff(T d) {
-    return d > T(0) ? d : (d == d ? T(0) : d);
-}
+// arithmetic minus the geometric call payoff. max(., 0) turns a NaN value into 0: tiles holding a
+// NaN value are marked by the path kernel instead (NanBits) and their sums set to NaN.
 __device__ __forceinline__ float payoff_of(float2 v, uint32_t kind, float K, bool) {
-    return call_payoff((kind == kEuropean ? v.y : v.x) - K);
+    return __builtin_fmaxf((kind == kEuropean ? v.y : v.x) - K, 0.0f);
 }
 __device__ __forceinline__ float payoff_of(float4 v, uint32_t kind, float K, bool) {
-    const float pg = call_payoff(v.z - K);
-    const float pa = call_payoff((kind == kEuropean ? v.y : v.x) - K);
+    const float pg = __builtin_fmaxf(v.z - K, 0.0f);
+    const float pa = __builtin_fmaxf((kind == kEuropean ? v.y : v.x) - K, 0.0f);
     return kind == kGeoAsian ? pg : kind == kAsianGeoCV ? pa - pg : pa;
 }
 __device__ __forceinline__ double payoff_of(double4 v, uint32_t kind, double K, bool valid) {
-    const double pg = call_payoff(v.z - K);
-    const double pa = call_payoff((kind == kEuropean ? v.y : v.x) - K);
+    const double pg = __builtin_fmax(v.z - K, 0.0);
+    const double pa = __builtin_fmax((kind == kEuropean ? v.y : v.x) - K, 0.0);
     const double pay = kind == kGeoAsian ? pg : kind == kAsianGeoCV ? pa - pg : pa;
     return valid ? pay : 0.0;
 }
 
+// NaN marks of a wave's tiles at one expiry, wave-uniform: bit 3j + c is set when a path of tile j
+// (within n_paths) has a NaN value in component c (0: A, 1: X_T, 2: G). A path whose price left the
+// floating-point range (inf - inf in the finalisation) has one; the sums of every output of the
+// tile that reads that component are then NaN, as the NaN payoff would have made them, instead of
+// silently dropping the path.
+struct NanBits {
+    uint32_t bits = 0;
+    __device__ __forceinline__ void mark(int j, bool valid, float A, float xt, float G) {
+        const bool na = valid && A != A, nx = valid && xt != xt, ng = valid && G != G;
+        bits |= (__builtin_amdgcn_ballot_w64(na) ? 1u : 0u) << (3 * j);
+        bits |= (__builtin_amdgcn_ballot_w64(nx) ? 2u : 0u) << (3 * j);
+        bits |= (__builtin_amdgcn_ballot_w64(ng) ? 4u : 0u) << (3 * j);
+    }
+    __device__ __forceinline__ void mark(int j, bool valid, double A, double xt, double G) {
+        const bool na = valid && A != A, nx = valid && xt != xt, ng = valid && G != G;
+        bits |= (__builtin_amdgcn_ballot_w64(na) ? 1u : 0u) << (3 * j);
+        bits |= (__builtin_amdgcn_ballot_w64(nx) ? 2u : 0u) << (3 * j);
+        bits |= (__builtin_amdgcn_ballot_w64(ng) ? 4u : 0u) << (3 * j);
+    }
+    // whether outputs of payoff `kind` of tile j read a NaN value
+    __device__ __forceinline__ bool hit(int j, uint32_t kind) const {
+        const uint32_t c = kind == kEuropean ? 2u : kind == kGeoAsian ? 4u : kind == kAsianGeoCV ? 5u : 1u;
+        return ((bits >> (3 * j)) & c) != 0u;
+    }
+};
+
+// Stores a wave's PPT tile sums of output o: the chunk tree's level for PPT tiles (1, 2 or 4), or,
+// for 3 paths per thread (which does not divide a chunk), every tile's own sum, from which the
+// group reduction builds the same tree.
+template <int PPT, typename Acc>
+__device__ __forceinline__ void store_tile_sums(const PathArgs& a, const DevReq& R, uint32_t tile0, uint32_t n_out,
+                                                uint32_t o, bool active, TileSums<Acc> (&t)[PPT]) {
+    if constexpr (kChunkTiles % PPT == 0) {
+        const TileSums<Acc> c = chunk_level<PPT, Acc>(t);
+        if (active) c.store(a.partials, (size_t)R.partial_base + (size_t)(tile0 / PPT) * n_out + o);
+    } else {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j)
+            if (active && tile0 + (uint32_t)j < R.n_tiles)
+                t[j].store(a.partials, (size_t)R.partial_base + (size_t)(tile0 + (uint32_t)j) * n_out + o);
+    }
+}
+
 template <int PPT, typename ValT, typename Acc>
 __device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevReq& R, const DevFin& F,
-                                                  uint32_t tile0, int lane, const ValT* vals) {
+                                                  uint32_t tile0, int lane, const ValT* vals, NanBits nan) {
     using KT = decltype(ValT{}.x);
     const uint32_t nk = F.n_strikes;
     const uint32_t n_out = R.n_opts * R.n_pay;
@@ -225,9 +265,9 @@ __device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevRe
                     const uint64_t nv = R.n_paths - (uint64_t)tile * kTile;
                     for (int p = 0; p < kTile; ++p)
                         t[j].add((Acc)payoff_of(vals[j * kTile + p], kind, K, (uint64_t)p < nv));
+                    if (nan.hit(j, kind)) t[j].poison();
                 }
-                const TileSums<Acc> c = chunk_level<PPT, Acc>(t);
-                if (active) c.store(a.partials, (size_t)R.partial_base + (size_t)(tile0 / PPT) * n_out + o);
+                store_tile_sums<PPT, Acc>(a, R, tile0, n_out, o, active, t);
             }
         } else {
             const int g = lane & 7;
@@ -249,9 +289,9 @@ __device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevRe
                         t[j].add((Acc)payoff_of(vals[j * kTile + p], kind, K, (uint64_t)p < nv));
                     }
                     t[j].tree8();  // every lane of the 8-lane group ends with the same bits
+                    if (nan.hit(j, kind)) t[j].poison();
                 }
-                const TileSums<Acc> c = chunk_level<PPT, Acc>(t);
-                if (g == 0 && active) c.store(a.partials, (size_t)R.partial_base + (size_t)(tile0 / PPT) * n_out + o);
+                store_tile_sums<PPT, Acc>(a, R, tile0, n_out, o, g == 0 && active, t);
             }
         }
     }
@@ -259,11 +299,11 @@ __device__ __forceinline__ void reduce_tiles_impl(const PathArgs& a, const DevRe
 
 template <int PPT, typename ValT>
 __device__ __forceinline__ void reduce_tiles(const PathArgs& a, const DevReq& R, const DevFin& F, uint32_t tile0,
-                                             int lane, const ValT* vals, bool) {
+                                             int lane, const ValT* vals, NanBits nan) {
     if (std::is_same<decltype(ValT{}.x), float>::value && R.precision == kPrecComp)
-        reduce_tiles_impl<PPT, ValT, float>(a, R, F, tile0, lane, vals);
+        reduce_tiles_impl<PPT, ValT, float>(a, R, F, tile0, lane, vals, nan);
     else
-        reduce_tiles_impl<PPT, ValT, double>(a, R, F, tile0, lane, vals);
+        reduce_tiles_impl<PPT, ValT, double>(a, R, F, tile0, lane, vals, nan);
 }
 
 __device__ __forceinline__ void count_flush(unsigned long long* counts, uint32_t c0, uint32_t c1, uint32_t c2,
@@ -292,46 +332,12 @@ struct SegF {
     float ope;
 };
 
-// Per-wave LDS scratch of the FP32 kernel: the tail-normal queue during the steps (2*PPT words in,
-// 2*PPT results out per lane) and the (A, X_T) transpose at expiries, which reuses the same space.
+// Per-wave LDS scratch of the FP32 kernel: the transpose of the per-path values at expiries, 4 words
+// per path ((A, X_T) or, with the geometric payoffs, (A, X_T, G, -)).
 template <int PPT>
 struct WaveLds {
-    static constexpr int kWords = 2 * (2 * PPT * kTile);  // queue in + queue out, 32-bit words
+    static constexpr int kWords = 4 * PPT * kTile;  // 32-bit words
 };
-
-// AS241 tail evaluations gathered across the wave. Each lane holds N normals whose central-region
-// value is already in z[s]; where t[s] is set (|q| > 0.425) the tail rational is needed instead.
-// The wave packs those words into an LDS queue, evaluates the queue densely 64 entries at a time,
-// and each lane reads its results back. With 15% tail draws this replaces N full-wave tail
-// evaluations by ceil(0.15 * 64 N / 64) dense ones. A word's slot is the wave-uniform count of
-// earlier entries plus its lane's rank among the wave's lanes with a tail draw (mbcnt over the
-// ballot); the uniform part stays in a scalar register and enters the LDS address as a base, so
-// a slot costs three vector instructions (two mbcnt, one shift-add), and its result is read back
-// from the same address at a fixed offset. q must be wave-uniform.
-template <int N>
-__device__ __forceinline__ void tail_normals(const uint32_t (&w)[N], const bool (&t)[N], const uint64_t (&mk)[N],
-                                             float (&z)[N], uint32_t* q, int lane, const Ppnd7Vregs& pk) {
-    uint32_t rank[N], base[N];
-    uint32_t total = 0;  // wave-uniform
-#pragma unroll
-    for (int s = 0; s < N; ++s) {
-        rank[s] = __builtin_amdgcn_mbcnt_hi((uint32_t)(mk[s] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[s], 0u));
-        base[s] = total;
-        if (t[s]) q[base[s] + rank[s]] = w[s];
-        total += (uint32_t)__builtin_popcountll(mk[s]);
-    }
-    float* q_out = reinterpret_cast<float*>(q + N * kTile);
-    wave_sync_lds();
-    for (uint32_t p = 0; p < total; p += kTile) {
-        const uint32_t i = p + (uint32_t)lane;
-        if (i < total) q_out[i] = normal_tail_f32(q[i], pk);
-    }
-    wave_sync_lds();
-#pragma unroll
-    for (int s = 0; s < N; ++s)
-        if (t[s]) z[s] = q_out[base[s] + rank[s]];
-    wave_sync_lds();  // the queue is rewritten by the next step
-}
 
 struct StepCount {
     uint32_t quad = 0, exp = 0, central = 0, tail = 0;
@@ -403,7 +409,8 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
             const float lg = __builtin_amdgcn_logf(tm2 * rcp_refined(lhs));
             const float ib = sm * rcp_refined(m[j]);  // 2 / beta
             const float v = ib * (0.5f * 0.693147180559945309f) * lg;
-            dm = (lhs < tm2 ? v : 0.0f) - m[j];
+            // U > p exactly when (1-p)/(1-U) > 1, i.e. when the log is positive: V' = max(v, 0)
+            dm = __builtin_fmaxf(v, 0.0f) - m[j];
             if constexpr (MART) {
                 // cA = A / beta < 1, r1 = (1-p) / (1 - cA) = 2 m^2 / (sm (1 - cA)), y = r1 cA:
                 // A m - ln E[exp(A V')] = f(y) - r1 cA^2
@@ -451,29 +458,8 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
     }
 }
 
-
-// The normals of one step of path j, in queue slots s0 (Z_V) and s0 + 1 (Z_X): their central-region
-// values and whether the tail rational is needed instead (for Z_V only where the step's variance
-// branch is quadratic: need_v). tv / tx: the words' own tail tests (branch counts).
-// mk: the wave's ballots of t, formed from the ballots of the comparisons themselves (the ballot of
-// a combined bool goes through a vector register: two more instructions).
-template <int N>
-__device__ __forceinline__ void queue_normals(uint32_t wv, uint32_t wx, bool need_v, int s0, float (&z)[N],
-                                              uint32_t (&w)[N], bool (&t)[N], uint64_t (&mk)[N], bool& tv,
-                                              bool& tx, const Ppnd7Vregs& pk) {
-    const float qv = central_q_f32(wv), qx = central_q_f32(wx);
-    z[s0] = normal_central_rational_f32(qv, pk);
-    z[s0 + 1] = normal_central_rational_f32(qx, pk);
-    w[s0] = wv;
-    w[s0 + 1] = wx;
-    tv = __builtin_fabsf(qv) > 0.425f;
-    tx = __builtin_fabsf(qx) > 0.425f;
-    t[s0] = need_v && tv;
-    t[s0 + 1] = tx;
-    mk[s0] = __builtin_amdgcn_ballot_w64(need_v) & __builtin_amdgcn_ballot_w64(tv);
-    mk[s0 + 1] = __builtin_amdgcn_ballot_w64(tx);
-}
-
+// Branch statistics (COUNT): quad / exp per step; central / tail per normal drawn (Z_V only where
+// the variance branch is quadratic).
 __device__ __forceinline__ void count_step(StepCount& cnt, bool valid, bool quad, bool tv, bool tx) {
     if (!valid) return;
     cnt.quad += quad ? 1u : 0u;
@@ -483,32 +469,23 @@ __device__ __forceinline__ void count_step(StepCount& cnt, bool valid, bool quad
 }
 
 // One QE step for the PPT paths of a lane (Andersen's QE scheme as in src/HSimulation.tpp:52-86):
-// the normals Z_V (quadratic lanes) and Z_X from AS241 with the tails gathered per wave, then the
-// update. Y is log2(X/S0); X = 2^Y; P accumulates X for the Asian sum.
+// the normals Z_V (quadratic lanes only: the exponential branch reads the word as a uniform) and
+// Z_X, then the update. Y is log2(X/S0); X = 2^Y; P accumulates X for the Asian sum.
 template <int PPT, bool COUNT, bool MART, bool GEO>
 __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], float (&X)[PPT], float (&Xp)[PPT],
                                              float (&P)[PPT], float (&Yp)[PPT], float (&PY)[PPT],
                                              const uint32_t (&wv)[PPT], const uint32_t (&wx)[PPT],
-                                             const SegF& c, uint32_t* q, int lane, const bool (&valid)[PPT],
-                                             StepCount& cnt, const Ppnd7Vregs& pk) {
+                                             const SegF& c, const bool (&valid)[PPT], StepCount& cnt,
+                                             const NormalVregs& nv) {
     float d0[PPT], m[PPT], s2h[PPT], m2[PPT], tt[PPT], zv[PPT], zx[PPT];
     bool quad[PPT];
-    uint32_t w[2 * PPT];
-    bool t[2 * PPT];
-    uint64_t mk[2 * PPT];
-    float z[2 * PPT];
     qe_moments_fp32<PPT>(V, c, d0, m, s2h, m2, tt, quad);
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         bool tv, tx;
-        queue_normals<2 * PPT>(wv[j], wx[j], quad[j], 2 * j, z, w, t, mk, tv, tx, pk);
+        zv[j] = normal_word_f32(wv[j], quad[j], nv, tv);
+        zx[j] = normal_word_f32(wx[j], true, nv, tx);
         if constexpr (COUNT) count_step(cnt, valid[j], quad[j], tv, tx);
-    }
-    tail_normals<2 * PPT>(w, t, mk, z, q, lane, pk);
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-        zv[j] = z[2 * j];
-        zx[j] = z[2 * j + 1];
     }
     qe_update_fp32<PPT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, wv, zv, zx, d0, m, s2h, m2, tt, quad, c);
 }
@@ -556,7 +533,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
         Yp[j] = PY[j] = accY[j] = Ys[j] = 0.0f;
         sv[j] = sx[j] = 0u;
     }
-    const Ppnd7Vregs pk = ppnd7_vregs();
+    const NormalVregs nv = normal_vregs();
     uint32_t k = 0;  // global step index; step k uses words 2k, 2k+1 of the path's stream
     const uint32_t n_segs = R.n_segs;
     for (uint32_t s = 0; s < n_segs; ++s) {
@@ -565,20 +542,20 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
         const uint32_t n = S.n_steps;
         uint32_t i = 0;
         if ((k & 1u) && n > 0) {  // second half of a block opened by the previous run
-            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, sv, sx, c, lds, lane, valid, cnt, pk);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, sv, sx, c, valid, cnt, nv);
             ++i;
             ++k;
         }
         for (; i + 2 <= n; i += 2, k += 2) {
             uint32_t w0[PPT], w1[PPT], w2[PPT], w3[PPT];
             philox_block<PPT>(k >> 1, plo, phi, key0, key1, w0, w1, w2, w3);
-            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w0, w1, c, lds, lane, valid, cnt, pk);
-            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w2, w3, c, lds, lane, valid, cnt, pk);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w0, w1, c, valid, cnt, nv);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w2, w3, c, valid, cnt, nv);
         }
         if (i < n) {
             uint32_t w0[PPT], w1[PPT];
             philox_block<PPT>(k >> 1, plo, phi, key0, key1, w0, w1, sv, sx);
-            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w0, w1, c, lds, lane, valid, cnt, pk);
+            qe_step_fp32<PPT, COUNT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, w0, w1, c, valid, cnt, nv);
             ++k;
         }
         // Asian accumulation of the run's interior steps b..e-1:
@@ -588,13 +565,14 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
             acc[j] = __builtin_fmaf(S.delta_f, (P[j] - X[j]) + 0.5f * (Xs[j] - Xp[j]), acc[j]);
             if constexpr (GEO) accY[j] = __builtin_fmaf(S.delta_f, (PY[j] - Y[j]) + 0.5f * (Ys[j] - Yp[j]), accY[j]);
         }
-        // per-path values at an expiry: (A, X_T), with GEO (A, X_T, G, -); the wave's LDS queue
+        // per-path values at an expiry: (A, X_T), with GEO (A, X_T, G, -); the wave's LDS region
         // holds 4 words per path either way
         using Val = typename std::conditional<GEO, float4, float2>::type;
         Val* vals = reinterpret_cast<Val*>(lds);
         for (uint32_t f = 0; f < S.fin_count; ++f) {
             const DevFin& F = a.fins[S.fin_begin + f];
             const float w = F.w_f, invT = F.invT_f;
+            NanBits nan;
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
                 const float dX = X[j] - Xp[j];
@@ -611,8 +589,10 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
                     const float G = F.gW_f * __builtin_amdgcn_exp2f(L * F.gInvW_f);
                     vals[j * kTile + lane] =
                         valid[j] ? make_float4(A, xt, G, 0.0f) : make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+                    nan.mark(j, valid[j], A, xt, G);
                 } else {
                     vals[j * kTile + lane] = valid[j] ? make_float2(A, xt) : make_float2(-INFINITY, -INFINITY);
+                    nan.mark(j, valid[j], A, xt, 0.0f);
                 }
                 if constexpr (DEBUG) {
                     if (valid[j] && a.finals) {
@@ -624,7 +604,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
                 }
             }
             wave_sync_lds();
-            reduce_tiles<PPT>(a, R, F, tile0, lane, vals, true);
+            reduce_tiles<PPT>(a, R, F, tile0, lane, vals, nan);
             wave_sync_lds();
         }
         const float hn = 0.5f * S.dnext_f;
@@ -644,7 +624,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
 }
 
 template <int PPT, bool DEBUG, bool COUNT, bool MART, bool GEO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_path_fp32(PathArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PPT == 2 ? 5 : 4, 8))) void k_path_fp32(PathArgs a) {
     extern __shared__ uint32_t lds_u32[];
     const int lane = (int)(threadIdx.x & 63u);
     // the wave's LDS region, at a wave-uniform address
@@ -767,6 +747,8 @@ __device__ __forceinline__ void path_fp64(const PathArgs& a, uint32_t ri, uint32
                 G = F.gW * R.S0 * ::exp(L / F.gW);
             }
             vals[lane] = make_double4(A, xt, G, 0.0);
+            NanBits nan;
+            nan.mark(0, valid, A, xt, G);
             if constexpr (DEBUG) {
                 if (valid && a.finals) {
                     double* out = a.finals + (pl * R.n_exp + F.chain) * 2;
@@ -775,7 +757,7 @@ __device__ __forceinline__ void path_fp64(const PathArgs& a, uint32_t ri, uint32
                 }
             }
             wave_sync_lds();
-            reduce_tiles<1>(a, R, F, tile, lane, vals, false);
+            reduce_tiles<1>(a, R, F, tile, lane, vals, nan);
             wave_sync_lds();
         }
         acc += S.dnext * 0.5 * (X + Xp);
@@ -1045,6 +1027,7 @@ const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode, 
     switch (ppt) {
         HEXO_PPT_CASE(1)
         HEXO_PPT_CASE(2)
+        HEXO_PPT_CASE(3)
         HEXO_PPT_CASE(4)
         default: return nullptr;
     }
@@ -1054,8 +1037,8 @@ const void* path_kernel_symbol(uint32_t precision, uint32_t ppt, PathMode mode, 
 
 size_t path_lds_bytes(uint32_t precision, uint32_t ppt, uint32_t block) {
     const size_t waves = block / kTile;
-    // FP32: per wave the tail-normal queue (2 * 2 * ppt * 64 words), reused by the transpose of the
-    // per-path values at expiries (up to 4 words per path); FP64: 64 double4 per wave
+    // FP32: per wave the transpose of the per-path values at expiries (4 words per path, WaveLds);
+    // FP64: 64 double4 per wave
     return precision == kPrecFP64 ? waves * kTile * sizeof(double4) : waves * 4 * ppt * kTile * sizeof(uint32_t);
 }
 
@@ -1089,6 +1072,7 @@ hipError_t launch_paths(uint32_t precision, uint32_t ppt, PathMode mode, uint32_
     switch (ppt) {
         case 1: HEXO_LAUNCH(1) break;
         case 2: HEXO_LAUNCH(2) break;
+        case 3: HEXO_LAUNCH(3) break;
         case 4: HEXO_LAUNCH(4) break;
         default: return hipErrorInvalidValue;
     }

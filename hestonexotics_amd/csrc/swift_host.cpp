@@ -22,7 +22,7 @@ constexpr double kPi = 3.14159265358979323846;
 
 // The reference's characteristic function (src/HDistribution.cpp:12-38), for the truncation test.
 cd heston_chf(const double p[5], double tau, double u) {
-    const double v0 = p[0], vm = p[1], rho = p[2], kappa = p[3], sigma = p[4];
+    const double v0 = p[0], theta = p[1], rho = p[2], kappa = p[3], sigma = p[4];
     const cd iu(0.0, u);
     const cd xi = kappa + sigma * rho * iu;
     const cd fac = u * u - iu;
@@ -30,7 +30,7 @@ cd heston_chf(const double p[5], double tau, double u) {
     const cd sh = std::sinh(d * (tau * 0.5)), ch = std::cosh(d * (tau * 0.5));
     const cd A = fac * sh / (d / v0 * ch + xi / v0 * sh);
     const cd D = std::log(2.0 * d / (d + xi + (d - xi) * std::exp(-d * tau))) + (kappa - d) * (tau * 0.5);
-    return std::exp(kappa * vm * rho * tau * iu / sigma - A + 2.0 * D * kappa * vm / (sigma * sigma));
+    return std::exp(kappa * theta * rho * tau * iu / sigma - A + 2.0 * D * kappa * theta / (sigma * sigma));
 }
 
 // Truncation error of scale m (HDistribution::int_error, src/HDistribution.cpp:8-11).
@@ -40,34 +40,43 @@ double int_error(const double p[5], double tau, unsigned m) {
 }
 
 // Cumulants that size the truncation range (HDistribution::first/second/fourth_order_moment,
-// src/HDistribution.cpp:86-113). The reference binds p = (v0, theta, rho, kappa, sigma) to the
-// names (v, s2, r, kappa, k) there; the formulas keep that binding.
+// src/HDistribution.cpp:86-113), in the model's parameter names: p = (v0, theta, rho, kappa, sigma).
 double cumulant1(const double p[5], double t) { return -0.5 * p[1] * t; }
 double cumulant2(const double p[5], double t) {
-    const double s2 = p[1], r = p[2], a = p[3], k = p[4];
-    const double a2 = a * a, a3 = a2 * a, k2 = k * k;
-    return s2 / (8 * a3) *
-           (-k2 * std::exp(-2 * a * t) + 4 * k * std::exp(-a * t) * (k - 2 * a * r) +
-            2 * a * t * (4 * a2 + k2 - 4 * a * k * r) + k * (8 * a * r - 3 * k));
+    const double theta = p[1], rho = p[2], kappa = p[3], sigma = p[4];
+    const double kappa2 = kappa * kappa, kappa3 = kappa2 * kappa, sigma2 = sigma * sigma;
+    return theta / (8 * kappa3) *
+           (-sigma2 * std::exp(-2 * kappa * t) + 4 * sigma * std::exp(-kappa * t) * (sigma - 2 * kappa * rho) +
+            2 * kappa * t * (4 * kappa2 + sigma2 - 4 * kappa * sigma * rho) + sigma * (8 * kappa * rho - 3 * sigma));
 }
 double cumulant4(const double p[5], double t) {
-    const double s2 = p[1], r = p[2], a = p[3], k = p[4];
-    const double a2 = a * a, a3 = a2 * a, a4 = a3 * a, k2 = k * k, k3 = k2 * k, k4 = k3 * k;
-    const double t2 = t * t, r2 = r * r, q = k - 2 * a * r;
-    const double e1 = std::exp(-a * t), e2 = std::exp(-2 * a * t), e3 = std::exp(-3 * a * t), e4 = std::exp(-4 * a * t);
-    const double b4 = -3 * k4 * e4;
-    const double b3 = -8 * k2 * e3 * (2 * a * k * t * q + 4 * a2 + k2 - 6 * a * k * r);
-    const double b2 = -4 * e2 * (4 * a2 * k2 * t2 * q * q + 2 * a * k * t * (k3 - 16 * a3 * r - 12 * a * k2 * r + 4 * a2 * k * (3 + 4 * r2)) +
-                                 8 * a4 - 3 * k4 - 32 * a3 * k * r + 8 * a * k3 * r + 16 * a2 * k2 * r2);
-    const double b1 = -8 * e1 * (-2 * a2 * k * t2 * q * q * q -
-                                 8 * a * t * (k4 - 7 * a * k3 * r + 4 * a4 * r2 - 8 * a3 * k * r * (1 + r2) + a2 * k2 * (3 + 14 * r2)) -
-                                 9 * k4 + 70 * a * k3 * r + 32 * a3 * k * r * (4 + 3 * r2) - 16 * a4 * (1 + 4 * r2) -
-                                 4 * a2 * k2 * (9 + 40 * r2));
-    const double b0 = 4 * a * t * (5 * k4 - 40 * a * k3 * r - 32 * a3 * k * r * (3 + 2 * r2) + 16 * a4 * (1 + 4 * r2) +
-                                   24 * a2 * k2 * (1 + 4 * r2)) -
-                      73 * k4 + 544 * a * k3 * r + 128 * a3 * k * r * (7 + 6 * r2) - 32 * a4 * (3 + 16 * r2) -
-                      64 * a2 * k2 * (4 + 19 * r2);
-    return (3 * k2 * s2) / (64 * std::pow(a, 7)) * (b4 + b3 + b2 + b1 + b0);
+    const double theta = p[1], rho = p[2], kappa = p[3], sigma = p[4];
+    const double kappa2 = kappa * kappa, kappa3 = kappa2 * kappa, kappa4 = kappa3 * kappa;
+    const double sigma2 = sigma * sigma, sigma3 = sigma2 * sigma, sigma4 = sigma3 * sigma;
+    const double t2 = t * t, rho2 = rho * rho, q = sigma - 2 * kappa * rho;
+    const double e1 = std::exp(-kappa * t), e2 = std::exp(-2 * kappa * t), e3 = std::exp(-3 * kappa * t),
+                 e4 = std::exp(-4 * kappa * t);
+    const double b4 = -3 * sigma4 * e4;
+    const double b3 = -8 * sigma2 * e3 * (2 * kappa * sigma * t * q + 4 * kappa2 + sigma2 - 6 * kappa * sigma * rho);
+    const double b2 =
+        -4 * e2 *
+        (4 * kappa2 * sigma2 * t2 * q * q +
+         2 * kappa * sigma * t *
+             (sigma3 - 16 * kappa3 * rho - 12 * kappa * sigma2 * rho + 4 * kappa2 * sigma * (3 + 4 * rho2)) +
+         8 * kappa4 - 3 * sigma4 - 32 * kappa3 * sigma * rho + 8 * kappa * sigma3 * rho + 16 * kappa2 * sigma2 * rho2);
+    const double b1 = -8 * e1 *
+                      (-2 * kappa2 * sigma * t2 * q * q * q -
+                       8 * kappa * t *
+                           (sigma4 - 7 * kappa * sigma3 * rho + 4 * kappa4 * rho2 -
+                            8 * kappa3 * sigma * rho * (1 + rho2) + kappa2 * sigma2 * (3 + 14 * rho2)) -
+                       9 * sigma4 + 70 * kappa * sigma3 * rho + 32 * kappa3 * sigma * rho * (4 + 3 * rho2) -
+                       16 * kappa4 * (1 + 4 * rho2) - 4 * kappa2 * sigma2 * (9 + 40 * rho2));
+    const double b0 = 4 * kappa * t *
+                          (5 * sigma4 - 40 * kappa * sigma3 * rho - 32 * kappa3 * sigma * rho * (3 + 2 * rho2) +
+                           16 * kappa4 * (1 + 4 * rho2) + 24 * kappa2 * sigma2 * (1 + 4 * rho2)) -
+                      73 * sigma4 + 544 * kappa * sigma3 * rho + 128 * kappa3 * sigma * rho * (7 + 6 * rho2) -
+                      32 * kappa4 * (3 + 16 * rho2) - 64 * kappa2 * sigma2 * (4 + 19 * rho2);
+    return (3 * sigma2 * theta) / (64 * std::pow(kappa, 7)) * (b4 + b3 + b2 + b1 + b0);
 }
 
 bool finite_pos(double x) { return std::isfinite(x) && x > 0.0; }

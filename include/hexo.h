@@ -109,7 +109,7 @@ typedef struct hexo_request {
 /* Launch shape of the FP32 path kernel. Results do not depend on it: every 64-path tile is
  * reduced on its own and tiles are combined in a fixed order. */
 typedef struct hexo_tuning {
-    uint32_t paths_per_thread; /* 1, 2 or 4 */
+    uint32_t paths_per_thread; /* 1, 2, 3 or 4 */
     uint32_t block_threads;    /* 64, 128 or 256 */
     uint32_t waves_per_simd;   /* persistent-grid occupancy cap, 1..8 (0 = the occupancy API's maximum) */
     uint32_t reserved;
@@ -204,9 +204,9 @@ int hexo_rng_fill(hexo_ctx* ctx, uint64_t seed, uint32_t stream, uint64_t path0,
                   uint64_t word0, uint64_t n_words, uint32_t* out);
 /* Philox4x32-10 on the GPU: out[4i..4i+3] = philox(ctr[4i..], key[2i..]). */
 int hexo_philox4x32_10(hexo_ctx* ctx, const uint32_t* ctr, const uint32_t* key, uint64_t n, uint32_t* out);
-/* Word -> N(0,1) through AS241 on the GPU: PPND16 in HEXO_FP64 arithmetic, or the FP32 kernel's
- * PPND7 (AS241's single-precision routine) for HEXO_FP32 (replaces RNG::setup_g with ppnd16,
- * src/RNG.cpp:34-43, src/as241.f90:15-119). */
+/* Word -> N(0,1) on the GPU: AS241 PPND16 in HEXO_FP64 arithmetic, or the FP32 kernel's inverse
+ * normal for HEXO_FP32 (polynomials in log2(s (1 - s)), s = min(u, 1 - u); within 1.1e-6 of PPND16)
+ * (replaces RNG::setup_g with ppnd16, src/RNG.cpp:34-43, src/as241.f90:15-119). */
 int hexo_normals(hexo_ctx* ctx, const uint32_t* words, uint64_t n, uint32_t precision, double* out);
 
 /* ---- launch shape ---- */

@@ -190,78 +190,38 @@ double orc_normal_f64_from_word(uint32_t k) {
     return orc_ppnd16_f64(orc_uniform_from_word(k), &f);
 }
 
-/* AS241's single-precision routine PPND7 (Wichura, Appl. Statist. 37(3), 1988; the reference
- * ships only PPND16, src/as241.f90:15-119). Same regions as PPND16 (|q| <= 0.425, r <= 5) with
- * lower-degree rationals, accurate to about 1 part in 10^7. The product's FP32 kernel uses it. */
-static const double P7_A[4] = {3.3871327179e0, 5.0434271938e1, 1.5929113202e2, 5.9109374720e1};
-static const double P7_B[4] = {1.0, 1.7895169469e1, 7.8757757664e1, 6.7187563600e1};
-static const double P7_C[4] = {1.4234372777e0, 2.7568153900e0, 1.3067284816e0, 1.7023821103e-1};
-static const double P7_D[3] = {1.0, 7.3700164250e-1, 1.2021132975e-1};
-static const double P7_E[4] = {6.6579051150e0, 3.0812263860e0, 4.2868294337e-1, 1.7337203997e-2};
-static const double P7_F[3] = {1.0, 2.4197894225e-1, 1.2258202635e-2};
+/* The FP32 kernel's normal (hestonexotics_amd/csrc/device.hpp, normal_word_f32), restated with
+ * libm's log2f and sqrtf where the kernel uses the hardware's: z = Phi^-1(u), u = (k + 1/2) 2^-32.
+ * The word is folded onto the lower half, j = min(k, ~k), so s = (j + 1/2) 2^-32 = min(u, 1 - u)
+ * keeps full relative precision in both tails; with t = s (1 - s) and l = log2 t, |z| = (1 - 2s) h(l)
+ * where h is a degree-9 polynomial in l + 6.5 for l >= -11 and a degree-8 polynomial in
+ * sqrt(-l) - 4.5305938720703125 below (Giles' erfinv variable, M. Giles, "Approximating the erfinv
+ * function", GPU Computing Gems Jade Edition, 2011, in base 2). The coefficients are the product's
+ * own least-squares fits; tests/test_oracle_normal_f32.py pins this function against the FP64
+ * PPND16 and scipy's ndtri. */
+static const float NF32_C[10] = {2.3366379737854004f,     -0.23548607528209686f,   -0.004091143608093262f,
+                                 0.00035097255022265017f, 6.123504863353446e-05f,  3.1161430342763197e-06f,
+                                 -2.392110900473199e-07f, -4.523901608877168e-08f, -7.092287312993051e-10f,
+                                 2.1350940460074241e-10f};
+static const float NF32_D[9] = {4.8360700607299805f,     1.2490957975387573f,      -0.011836512014269829f,
+                                0.0019035495351999998f,  -0.00013401612523011863f, -0.00016207653970923275f,
+                                0.00019050169794354588f, -0.00012945415801368654f, 3.693158214446157e-05f};
 
-void orc_ppnd7_coefficients(double out[19]) {
-    int n = 0;
-    for (int i = 0; i < 4; ++i) out[n++] = P7_A[i];
-    for (int i = 1; i < 4; ++i) out[n++] = P7_B[i];
-    for (int i = 0; i < 4; ++i) out[n++] = P7_C[i];
-    for (int i = 1; i < 3; ++i) out[n++] = P7_D[i];
-    for (int i = 0; i < 4; ++i) out[n++] = P7_E[i];
-    for (int i = 1; i < 3; ++i) out[n++] = P7_F[i];
-}
-
-static inline double horner(const double* c, int n, double r) {
-    double acc = c[n - 1];
-    for (int i = n - 2; i >= 0; --i) acc = acc * r + c[i];
-    return acc;
-}
-static inline float hornerf(const double* c, int n, float r) {
-    float acc = (float)c[n - 1];
-    for (int i = n - 2; i >= 0; --i) acc = acc * r + (float)c[i];
+static inline float horner_fmaf(const float* c, int n, float x) {
+    float acc = c[n - 1];
+    for (int i = n - 2; i >= 0; --i) acc = fmaf(acc, x, c[i]);
     return acc;
 }
 
-/* PPND7 evaluated in double, PPND16's region logic (src/as241.f90:85-118). */
-double orc_ppnd7_f64(double p, int* ifault) {
-    *ifault = 0;
-    double q = p - 0.5;
-    if (fabs(q) <= 0.425) {
-        double r = 0.180625 - q * q;
-        return q * horner(P7_A, 4, r) / horner(P7_B, 4, r);
-    }
-    double r = (q < 0.0) ? p : 1.0 - p;
-    if (r <= 0.0) {
-        *ifault = 1;
-        return 0.0;
-    }
-    r = sqrt(-log(r));
-    double z;
-    if (r <= 5.0) {
-        r -= 1.6;
-        z = horner(P7_C, 4, r) / horner(P7_D, 3, r);
-    } else {
-        r -= 5.0;
-        z = horner(P7_E, 4, r) / horner(P7_F, 3, r);
-    }
-    return (q < 0.0) ? -z : z;
-}
-
-/* The FP32 kernel's normal: PPND7 in float, driven by the integer. q = u - 1/2 =
- * (k - 2^31 + 1/2) 2^-32 and the tail argument min(u, 1-u) = (min(k, ~k) + 1/2) 2^-32 are formed
- * from k, so both tails keep full relative precision. 32-bit words never reach the far tail:
- * min(u, 1-u) >= 2^-33 gives r <= 4.78 < 5. */
 float orc_normal_f32_from_word(uint32_t k) {
-    int32_t sk = (int32_t)(k ^ 0x80000000u);
-    float q = ((float)sk + 0.5f) * 0x1p-32f;
-    if (fabsf(q) <= 0.425f) {
-        float r = 0.180625f - q * q;
-        return q * hornerf(P7_A, 4, r) / hornerf(P7_B, 4, r);
-    }
-    uint32_t m = (sk < 0) ? k : ~k;
-    float rr = ((float)m + 0.5f) * 0x1p-32f;
-    float r = sqrtf(-logf(rr)) - 1.6f;
-    float z = hornerf(P7_C, 4, r) / hornerf(P7_D, 3, r);
-    return (sk < 0) ? -z : z;
+    const uint32_t j = (k & 0x80000000u) ? ~k : k;
+    const float s = fmaf((float)j, 0x1p-32f, 0x1p-33f);
+    const float t = fmaf(-s, s, s);
+    const float l = log2f(t);
+    const float h = (l < -11.0f) ? horner_fmaf(NF32_D, 9, sqrtf(-l) - 4.5305938720703125f)
+                                 : horner_fmaf(NF32_C, 10, l + 6.5f);
+    const float pr = h * fmaf(-2.0f, s, 1.0f);
+    return (k & 0x80000000u) ? pr : -pr;
 }
 
 /* ------------------------------------------------------------------------------------------

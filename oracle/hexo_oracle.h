@@ -55,9 +55,7 @@ double orc_uniform_from_word(uint32_t k); /* (k + 1/2) * 2^-32, exact in double 
 double orc_ppnd16_f64(double p, int* ifault);     /* AS241 evaluated in double */
 double orc_ppnd16_asbuilt(double p, int* ifault); /* AS241 with the reference's REAL*4 arithmetic */
 double orc_normal_f64_from_word(uint32_t k);      /* ppnd16_f64(uniform(k)) */
-double orc_ppnd7_f64(double p, int* ifault);      /* AS241 PPND7 (single-precision routine) in double */
-float orc_normal_f32_from_word(uint32_t k);       /* PPND7 in FP32 from the integer (symmetric tails) */
-void orc_ppnd7_coefficients(double out[19]);      /* a0..a3,b1..b3,c0..c3,d1,d2,e0..e3,f1,f2 */
+float orc_normal_f32_from_word(uint32_t k);       /* the FP32 kernel's inverse normal of word k */
 void orc_as241_coefficients(double out[45]);      /* a0..a7,b1..b7,c0..c7,d1..d7,e0..e7,f1..f7 */
 
 /* ---- one QE step (reference form, FP64) ----

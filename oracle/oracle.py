@@ -29,7 +29,7 @@ def _bind(lib: C.CDLL) -> C.CDLL:
     lib.orc_rng_words.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, _u32p]
     lib.orc_uniform_from_word.argtypes = [C.c_uint32]
     lib.orc_uniform_from_word.restype = C.c_double
-    for f in (lib.orc_ppnd16_f64, lib.orc_ppnd16_asbuilt, lib.orc_ppnd7_f64):
+    for f in (lib.orc_ppnd16_f64, lib.orc_ppnd16_asbuilt):
         f.argtypes = [C.c_double, C.POINTER(C.c_int)]
         f.restype = C.c_double
     lib.orc_normal_f64_from_word.argtypes = [C.c_uint32]
@@ -37,7 +37,6 @@ def _bind(lib: C.CDLL) -> C.CDLL:
     lib.orc_normal_f32_from_word.argtypes = [C.c_uint32]
     lib.orc_normal_f32_from_word.restype = C.c_float
     lib.orc_as241_coefficients.argtypes = [_f64p]
-    lib.orc_ppnd7_coefficients.argtypes = [_f64p]
     lib.orc_qe_step.argtypes = [_f64p, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
                                 C.POINTER(C.c_double), C.c_uint32, C.c_uint32]
     lib.orc_qe_step.restype = C.c_int
@@ -130,18 +129,6 @@ def ppnd16_asbuilt(p: float) -> tuple[float, int]:
     f = C.c_int(0)
     z = load().orc_ppnd16_asbuilt(p, C.byref(f))
     return z, f.value
-
-
-def ppnd7_f64(p: float) -> tuple[float, int]:
-    f = C.c_int(0)
-    z = load().orc_ppnd7_f64(p, C.byref(f))
-    return z, f.value
-
-
-def ppnd7_coefficients() -> np.ndarray:
-    out = np.zeros(19, np.float64)
-    load().orc_ppnd7_coefficients(out)
-    return out
 
 
 def normals_f64(words) -> np.ndarray:

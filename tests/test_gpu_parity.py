@@ -53,13 +53,12 @@ def test_normals_fp64_and_fp32(ctx, oracle):
     c64 = oracle.normals_f64(w)
     np.testing.assert_allclose(g64, c64, rtol=2e-15, atol=1e-15)
     g32 = ctx.normals(w, H.FP32)
-    # FP32 evaluation (AS241 PPND7): within 4e-6 relative of the FP64 PPND16 value (PPND7 ~1e-7,
-    # tail argument from the integer, v_log/v_sqrt/v_rcp approximations) ...
-    np.testing.assert_allclose(g32, c64, rtol=4e-6, atol=4e-7)
-    # ... and within a few FP32 ulps of the oracle's PPND7 restatement (libm logf/sqrtf and IEEE
-    # division there, hardware approximations and FMA chains here)
+    # FP32 evaluation (polynomials in log2(s (1 - s)), hardware log2 and sqrt): within 1e-6 of the
+    # FP64 PPND16 value where |z| < 3.3 and 4e-7 relative beyond ...
+    np.testing.assert_allclose(g32, c64, rtol=4e-7, atol=1e-6)
+    # ... and within a few FP32 ulps of the oracle's restatement (libm log2f/sqrtf there)
     c32 = oracle.normals_f32(w).astype(np.float64)
-    np.testing.assert_allclose(g32, c32, rtol=1.5e-6, atol=2e-7)
+    np.testing.assert_allclose(g32, c32, rtol=2e-7, atol=2e-7)
 
 
 def small_requests():
@@ -196,12 +195,16 @@ def test_batch_equals_single(ctx):
 
 
 def test_tuning_does_not_change_results(ctx):
+    """Every launch shape gives the same bits: 1-4 paths per thread (3 stores per-tile sums, the
+    others the 4-tile chunk tree's level; the group reduction completes the tree), with few strikes
+    (the 8-path-group reduction) and many (one lane per strike), and a path count that leaves the
+    last chunk and the last tile partial."""
     say("gpu tuning invariance")
-    reqs = W.c2(n_paths=30_000, maturities=[0.5, 2.0], n_strikes=8)
+    reqs = W.c2(n_paths=30_000, maturities=[0.5, 2.0], n_strikes=8) + W.c2(n_paths=20_000, maturities=[1.0])
     base = ctx.get_tuning()
     ref = ctx.price_batch(reqs)
     try:
-        for ppt, blk, w in [(1, 64, 0), (4, 256, 2), (2, 128, 4)]:
+        for ppt, blk, w in [(1, 64, 0), (4, 256, 2), (2, 128, 4), (3, 256, 0), (3, 64, 4)]:
             ctx.set_tuning(ppt, blk, w)
             got = ctx.price_batch(reqs)
             for (p, s), (q, t) in zip(ref, got):
@@ -294,7 +297,7 @@ def test_autotuner_picks_a_valid_shape_and_keeps_results(ctx):
     ref = ctx.price_batch(reqs)
     try:
         best = ctx.autotune(reqs, reps=1)
-        assert best["paths_per_thread"] in (1, 2, 4) and best["block_threads"] in (64, 128, 256)
+        assert best["paths_per_thread"] in (1, 2, 3, 4) and best["block_threads"] in (64, 128, 256)
         assert ctx.get_tuning() == best
         for (p, s), (q, t) in zip(ref, ctx.price_batch(reqs)):
             np.testing.assert_array_equal(p, q)

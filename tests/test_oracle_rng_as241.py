@@ -2,8 +2,7 @@
 
 Philox: the Random123 known-answer vectors and rocRAND's philox4x32_10 engine (tests/golden).
 AS241: scipy.special.ndtri (a library inverse normal), the coefficient hash sums printed in the
-algorithm's published listing (src/as241.f90:45,64,83 for PPND16; Wichura 1988 for PPND7, the
-single-precision routine the FP32 kernel uses), symmetry and the IFAULT cases.
+algorithm's published listing (src/as241.f90:45,64,83), symmetry and the IFAULT cases.
 """
 import json
 import os
@@ -84,42 +83,6 @@ def mant(x):
     return x / 10.0 ** np.floor(np.log10(abs(x)))
 
 
-def test_ppnd7_coefficient_hash_sums():
-    """PPND7's hash sums as printed with the published AS241 listing (Wichura 1988):
-    AB 32.3184577772, CD 15.7614929821, EF 19.4052910204."""
-    c = O.ppnd7_coefficients()
-    assert abs(sum(mant(x) for x in c[0:7]) - 32.3184577772) < 1e-10
-    assert abs(sum(mant(x) for x in c[7:13]) - 15.7614929821) < 1e-10
-    assert abs(sum(mant(x) for x in c[13:19]) - 19.4052910204) < 1e-10
-
-
-def test_ppnd7_against_ndtri():
-    """PPND7 is accurate to about 1 part in 10^7 in every region (a wrong coefficient digit within
-    that accuracy shows up as a larger error), and it is genuinely the 7-digit routine."""
-    g = json.load(open(os.path.join(GOLD, "ndtri.json")))
-    p = np.array(g["p"])
-    z = np.array(g["z"])
-    ours = np.array([O.ppnd7_f64(x)[0] for x in p])
-    sel = np.abs(z) > 1e-3
-    rel = np.abs(ours[sel] - z[sel]) / np.abs(z[sel])
-    assert rel.max() < 3e-7
-    assert rel.max() > 1e-9
-    far = p < np.exp(-25.0)  # r > 5: the E/F rational
-    assert far.any() and (np.abs(ours[far] - z[far]) / np.abs(z[far])).max() < 3e-7
-    assert O.ppnd7_f64(0.5) == (0.0, 0) and O.ppnd7_f64(0.0) == (0.0, 1) and O.ppnd7_f64(1.0) == (0.0, 1)
-
-
-def test_ppnd7_fp32_matches_ppnd16_to_single_precision():
-    """The FP32 kernel's PPND7 normals and the reference's REAL*4 PPND16 (src/as241.f90:20-25)
-    agree to single precision on the same uniforms."""
-    rng = np.random.default_rng(11)
-    w = rng.integers(0, 2**32, 3000, dtype=np.uint64).astype(np.uint32)
-    a = O.normals_f32(w).astype(np.float64)
-    b = np.array([O.ppnd16_asbuilt(O.load().orc_uniform_from_word(int(k)))[0] for k in w])
-    sel = np.abs(b) > 1e-2
-    assert (np.abs(a[sel] - b[sel]) / np.abs(b[sel])).max() < 2e-6
-
-
 def test_as241_f64_against_ndtri():
     g = json.load(open(os.path.join(GOLD, "ndtri.json")))
     p = np.array(g["p"])
@@ -153,8 +116,9 @@ def test_as241_asbuilt_fp32_error_is_single_precision():
     assert rel.max() > 1e-9  # it really is single precision
 
 
-def test_as241_fp32_from_word_against_ndtri():
-    """The FP32 evaluation driven by the 32-bit word keeps full relative precision in both tails."""
+def test_fp32_normal_from_word_against_ndtri():
+    """The FP32 kernel's normal (its oracle restatement), driven by the 32-bit word, against scipy's
+    ndtri: within 8e-7 absolute, and full relative precision in both tails."""
     rng = np.random.default_rng(3)
     w = rng.integers(0, 2**32, 4000, dtype=np.uint64).astype(np.uint32)
     w[:4] = [0, 1, 2**32 - 2, 2**32 - 1]
@@ -162,10 +126,11 @@ def test_as241_fp32_from_word_against_ndtri():
     from scipy.special import ndtri
     z = ndtri(u)
     ours = O.normals_f32(w).astype(np.float64)
-    sel = np.abs(z) > 1e-3
-    assert (np.abs(ours[sel] - z[sel]) / np.abs(z[sel])).max() < 2e-6
-    # the extreme words reach |z| ~ 6.36 without losing precision (no 1 - u cancellation)
-    assert abs(ours[0] - z[0]) / abs(z[0]) < 1e-6 and abs(ours[3] - z[3]) / abs(z[3]) < 1e-6
+    assert np.abs(ours - z).max() < 8e-7
+    sel = np.abs(z) > 1.0
+    assert (np.abs(ours[sel] - z[sel]) / np.abs(z[sel])).max() < 3.5e-7
+    # the extreme words reach |z| ~ 6.34 without losing precision (no 1 - u cancellation)
+    assert abs(ours[0] - z[0]) / abs(z[0]) < 3e-7 and abs(ours[3] - z[3]) / abs(z[3]) < 3e-7
 
 
 def test_normal_moments():
