@@ -5,9 +5,9 @@
 //    computed directly, so paths can be split across waves, launches or GPUs freely.
 //  * Uniforms u = (k + 1/2) 2^-32 from a 32-bit word k (the reference maps a 64-bit word,
 //    src/RNG.cpp:31). u is never 0 or 1, and 1 - u(k) = u(~k) exactly.
-//  * AS241 inverse normal (Wichura 1988; src/as241.f90:15-119): an FP64 version in the
-//    reference's form and an FP32 version that forms q = u - 1/2 and min(u, 1-u) from the
-//    integer, evaluates the central and tail rationals without branching and selects.
+//  * Inverse normals: AS241 PPND16 (Wichura 1988; src/as241.f90:15-119) in FP64 in the reference's
+//    form, and the FP32 kernel's own: polynomials in log2(s (1 - s)) with s = min(u, 1-u) formed
+//    from the integer, the tail polynomial evaluated only where a lane needs it.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -120,7 +120,9 @@ __device__ __forceinline__ double normal_f64(uint32_t k) {
 // in vector registers: the VOP2 forms that encode a literal. A VOP3 v_fma_f32 cannot, so for a
 // constant operand the compiler reads a scalar register instead, and on gfx950 an FP32 instruction
 // with a scalar-register operand issues at about half the rate of one without (measured). The
-// result is the same fused multiply-add, bit for bit.
+// result is the same fused multiply-add, bit for bit. The compiler does not insert wait states in
+// front of inline asm: an operand must not be the result of a transcendental (v_log, v_exp,
+// v_sqrt, v_rcp, ...) issued just before, which needs one.
 template <uint32_t K>
 __device__ __forceinline__ float fmamk_lit(float a, float b) {
     float d;
@@ -155,18 +157,18 @@ __device__ __forceinline__ uint32_t in_vgpr_u32(uint32_t x) {
 // t = s (1 - s) and l = log2 t (in [-33, -2] for 32-bit words), |z| = (1 - 2s) h(l), where h is smooth
 // in l: Giles' variable for erfinv (M. Giles, "Approximating the erfinv function", GPU Computing
 // Gems Jade Edition, 2011: w = -ln(1 - x^2) with x = 1 - 2s, i.e. w = -(l + 2) ln 2). h is a degree-9
-// polynomial in l + 6.5 where l >= -11 (s >= 4.9e-4: all but about 1 draw in 1024) and a degree-8
+// polynomial in l where l >= -11 (s >= 4.9e-4: all but about 1 draw in 1024) and a degree-8
 // polynomial in sqrt(-l) - 4.5306 below. The coefficients are weighted least-squares fits of
-// -ndtri(s) / (1 - 2s) by normal_f32_fit.py (this directory). Evaluated in FP32 (hardware log2 and
-// square root, fused multiply-adds), the result is within 7e-7 of Phi^-1(u) for l >= -11 and within
-// 3e-7 relative below (tests/test_oracle_normal_f32.py pins the restatement in the oracle).
+// -ndtri(s) / (1 - 2s) by normal_f32_fit.py (this directory; the central fit is made in l + 6.5 and
+// expanded in powers of l in FP64). Evaluated in FP32 (hardware log2 and square root, fused
+// multiply-adds), the result is within 8e-7 of Phi^-1(u) for l >= -11 and within 3e-7 relative
+// below (tests/test_oracle_normal_f32.py pins the restatement in the oracle).
 struct NormalF32 {
     static constexpr float lc = -11.0f;  // the tail polynomial applies below
-    static constexpr float xc = 6.5f;    // central variable x = l + xc
-    static constexpr float c[10] = {2.3366379737854004f,     -0.23548607528209686f,    -0.004091143608093262f,
-                                    0.00035097255022265017f, 6.123504863353446e-05f,   3.1161430342763197e-06f,
-                                    -2.392110900473199e-07f, -4.523901608877168e-08f,  -7.092287312993051e-10f,
-                                    2.1350940460074241e-10f};
+    static constexpr float c[10] = {0.8369221091270447f,     -0.18630073964595795f,   0.011675846762955189f,
+                                    1.11724120870349e-05f,   -0.0002003998088184744f, -9.236554433300626e-06f,
+                                    1.7887376770886476e-06f, 2.4262888587145426e-07f, 1.1781071229677309e-08f,
+                                    2.1350940460074241e-10f};  // central: powers of l
     static constexpr float yc = 4.5305938720703125f;  // tail variable y = sqrt(-l) - yc
     static constexpr float d[9] = {4.8360700607299805f,      1.2490957975387573f,     -0.011836512014269829f,
                                    0.0019035495351999998f,   -0.00013401612523011863f, -0.00016207653970923275f,
@@ -191,16 +193,18 @@ __device__ __forceinline__ float normal_word_f32(uint32_t k, bool need, const No
     const float s = fmamk_lit<HEXO_F32_BITS(0x1p-32f)>((float)j, nv.h33);
     const float t = __builtin_fmaf(-s, s, s);
     const float l = __builtin_amdgcn_logf(t);
-    const float x = l + NormalF32::xc;
-    float h = fmamk_lit<HEXO_F32_BITS(NormalF32::c[9])>(x, nv.c8);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[7])>(h, x);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[6])>(h, x);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[5])>(h, x);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[4])>(h, x);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[3])>(h, x);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[2])>(h, x);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[1])>(h, x);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[0])>(h, x);
+    // The first Horner step is left to the compiler (it forms v_fmamk_f32 itself here): it reads
+    // the log2's result, and only the compiler inserts the wait state that a transcendental's
+    // result needs before the next instruction reads it; inline asm gets none.
+    float h = __builtin_fmaf(l, NormalF32::c[9], nv.c8);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[7])>(h, l);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[6])>(h, l);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[5])>(h, l);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[4])>(h, l);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[3])>(h, l);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[2])>(h, l);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[1])>(h, l);
+    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[0])>(h, l);
     tail = l < NormalF32::lc;
     if (need && tail) {
         const float y = __builtin_amdgcn_sqrtf(-l) - NormalF32::yc;

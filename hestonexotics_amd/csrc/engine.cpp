@@ -497,7 +497,6 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
             const double dl = S.delta, k = q.kappa, th = q.theta, e = q.sigma, rho = q.rho;
             const double dd = std::exp(-k * dl), omd = -std::expm1(-k * dl);
             g.omd = (float)omd;
-            g.ope = (float)(1.0 + dd);
             g.th = (float)th;
             // half the conditional variance's coefficients (the FP32 step forms s2 / 2)
             g.c1h = (float)(0.5 * e * e * dd * omd / k);
@@ -510,8 +509,9 @@ int batch_create(hexo_ctx* ctx, const hexo_request* reqs, uint32_t n, hipStream_
             // parts cancel exactly, leaving (r - th_f/2) dt. With the exact theta in K0 they would
             // leave rho kappa dt (th_f - theta) / sigma per step, 1e-5 at sigma = 1e-6.
             g.g0 = (float)((d.r_drift * dl - 0.5 * (double)g.th * dl) * L2E);
-            g.kc = (float)(kc * L2E);
-            g.kd = (float)(kd * L2E);
+            // K2 = kc + kd, K1 = kc - kd; kb = -(K1 + K2 e^-k dl) = kd omd - kc (1 + e^-k dl)
+            g.ka = (float)((kc + kd) * L2E);
+            g.kb = (float)((kd * omd - kc * (1.0 + dd)) * L2E);
             g.k3 = (float)(K3 * L2E * L2E);
             const double A = kc + kd + 0.5 * K3;  // K2 + K4/2
             g.mA = (float)A;

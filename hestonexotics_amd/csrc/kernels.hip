@@ -327,9 +327,8 @@ __device__ __forceinline__ void count_flush(unsigned long long* counts, uint32_t
 // FP32 path kernel
 // ------------------------------------------------------------------------------------------
 struct SegF {
-    float omd, th, c1h, c2h, g0, kc, kd, k3;
+    float omd, th, c1h, c2h, g0, ka, kb, k3;
     float mA, ma, mh, mr;
-    float ope;
 };
 
 // Per-wave LDS scratch of the FP32 kernel: the transpose of the per-path values at expiries, 4 words
@@ -425,12 +424,12 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
         // The reference's log-price step multiplies V' - V by rho/sigma and V + V' - 2 th by
         // kappa rho dt / (2 sigma). As differences of FP32 variances they would carry ulp(V) errors
         // amplified by 1/sigma (prices tens of percent off at sigma = 1e-6, or at sigma = 1e-4
-        // with dt = 2e-7). Formed from V' - m and th - V, each term has its own size:
-        // V' - V = (V' - m) + omd (th - V), V + V' - 2 th = (V' - m) - (1 + e^{-kappa dt}) (th - V).
+        // with dt = 2e-7). Written on V' - m and th - V, each term has its own size:
+        // K1 V + K2 V' = (K1 + K2) th + K2 (V' - m) - (K1 + K2 e^{-kappa dt}) (th - V), the first
+        // part in g0, the coefficients ka = K2 and kb (O(dt), formed in FP64) in the segment.
         const float Vn = m[j] + dm;
-        const float du = __builtin_fmaf(c.omd, d0[j], dm);
         const float sum = V[j] + Vn;
-        float inc = __builtin_fmaf(c.kd, du, __builtin_fmaf(c.kc, __builtin_fmaf(-c.ope, d0[j], dm), c.g0));
+        float inc = __builtin_fmaf(c.ka, dm, __builtin_fmaf(c.kb, d0[j], c.g0));
         if constexpr (MART) {
             // f(t) = t - ln(1 + t): its series through t^7 for |t| < 1/16 (relative error < 2e-8),
             // else from the rounded 1 + t. The direct form alone would return f = t wherever 1 + t
@@ -490,13 +489,60 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
     qe_update_fp32<PPT, MART, GEO>(V, Y, X, Xp, P, Yp, PY, wv, zv, zx, d0, m, s2h, m2, tt, quad, c);
 }
 
+// Philox4x32-10 (device.hpp philox4x32_10) of the path counter {blk, plo, phi, 0} with the step
+// pair blk, the path's high word phi and the key wave-uniform. The rounds are the same; the values
+// that stay wave-uniform through the first three rounds are marked so (readfirstlane), which keeps
+// their products on the scalar unit: both of round 1's, one each of rounds 2 and 3.
+__device__ __forceinline__ u32x4 philox_path(uint32_t blk, uint32_t plo, uint32_t phi, uint32_t k0, uint32_t k1) {
+    // round 1: c0 = blk, c2 = phi, c3 = 0 uniform
+    uint64_t p0 = (uint64_t)kPhiloxM0 * blk, p1 = (uint64_t)kPhiloxM1 * phi;
+    uint32_t c0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), plo, k0, 0x96);
+    uint32_t c1 = uniform((uint32_t)p1);
+    uint32_t c2 = uniform((uint32_t)(p0 >> 32)) ^ uniform(k1);
+    uint32_t c3 = uniform((uint32_t)p0);
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+    // round 2: c2 uniform; the new c0 and c1 are uniform
+    p0 = (uint64_t)kPhiloxM0 * c0;
+    p1 = (uint64_t)kPhiloxM1 * c2;
+    c0 = uniform((uint32_t)(p1 >> 32)) ^ c1 ^ uniform(k0);
+    c1 = uniform((uint32_t)p1);
+    c2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
+    c3 = (uint32_t)p0;
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+    // round 3: c0 uniform; the new c3 is uniform
+    p0 = (uint64_t)kPhiloxM0 * c0;
+    p1 = (uint64_t)kPhiloxM1 * c2;
+    c0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+    c1 = (uint32_t)p1;
+    c2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
+    c3 = uniform((uint32_t)p0);
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+#pragma unroll
+    for (int r = 3; r < 10; ++r) {
+        p0 = (uint64_t)kPhiloxM0 * c0;
+        p1 = (uint64_t)kPhiloxM1 * c2;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
+        c0 = n0;
+        c1 = (uint32_t)p1;
+        c2 = n2;
+        c3 = (uint32_t)p0;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    return u32x4{c0, c1, c2, c3};
+}
+
 template <int PPT>
 __device__ __forceinline__ void philox_block(uint32_t blk, const uint32_t (&plo)[PPT], const uint32_t (&phi)[PPT],
                                              uint32_t key0, uint32_t key1, uint32_t (&w0)[PPT], uint32_t (&w1)[PPT],
                                              uint32_t (&w2)[PPT], uint32_t (&w3)[PPT]) {
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        const u32x4 r = philox4x32_10(blk, plo[j], phi[j], 0u, key0, key1);
+        const u32x4 r = philox_path(blk, plo[j], phi[j], key0, key1);
         w0[j] = r.x;
         w1[j] = r.y;
         w2[j] = r.z;
@@ -538,7 +584,7 @@ __device__ __forceinline__ void path_fp32(const PathArgs& a, uint32_t ri, uint32
     const uint32_t n_segs = R.n_segs;
     for (uint32_t s = 0; s < n_segs; ++s) {
         const DevSeg& S = a.segs[R.seg_begin + s];
-        const SegF c{S.omd, S.th, S.c1h, S.c2h, S.g0, S.kc, S.kd, S.k3, S.mA, S.ma, S.mh, S.mr, S.ope};
+        const SegF c{S.omd, S.th, S.c1h, S.c2h, S.g0, S.ka, S.kb, S.k3, S.mA, S.ma, S.mh, S.mr};
         const uint32_t n = S.n_steps;
         uint32_t i = 0;
         if ((k & 1u) && n > 0) {  // second half of a block opened by the previous run

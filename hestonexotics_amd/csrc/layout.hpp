@@ -30,13 +30,16 @@ constexpr unsigned long long kMaxPaths = (1ull << 37) - kTile;  // per request: 
 // expiry finalisations). FP32 constants are computed on the host in FP64 (per request, per run):
 //   m  = V + omd (th - V)            (conditional mean, src/HSimulation.tpp:58-59; omd = 1 - e^-kappa dt)
 //   s2 = 2 (V c1h + c2h)             (conditional variance, src/HSimulation.tpp:60; halved constants)
-//   y += g0 + kc (V + V' - 2 th) + kd (V' - V) + sqrt(k3 (V + V')) Z      (log2 of X/S0)
-// with kc = (K1 + K2)/2, kd = (K2 - K1)/2 = rho/eps, k3 = K3 = K4 and g0 = K0 + 2 th kc
-// (src/HSimulation.tpp:75-80), all scaled by log2(e) (k3 by log2(e)^2); th is theta rounded to
-// FP32 and g0 uses that same value. This is the reference's step re-associated so that rounding
-// the constants to FP32 moves neither fixed point: in the form V e^-kappa dt + theta(1 - e^-kappa dt)
-// the rounded e^-kappa dt shifts the mean-reversion level by O(ulp / (kappa dt)), and K0 and
-// K1 V + K2 V' nearly cancel, so their separate roundings add a drift of O(ulp K0) per step.
+//   y += g0 + ka (V' - m) + kb (th - V) + sqrt(k3 (V + V')) Z              (log2 of X/S0)
+// with ka = K2, kb = -(K1 + K2 e^-kappa dt), k3 = K3 = K4 and g0 = K0 + (K1 + K2) th
+// (src/HSimulation.tpp:75-80: K1 V + K2 V' = (K1 + K2) th + K2 (V' - m) - (K1 + K2 e^-kappa dt)(th - V)),
+// all scaled by log2(e) (k3 by log2(e)^2); th is theta rounded to FP32 and g0 uses that same
+// value. This is the reference's step re-associated so that rounding the constants to FP32 moves
+// neither fixed point: in the form V e^-kappa dt + theta(1 - e^-kappa dt) the rounded e^-kappa dt
+// shifts the mean-reversion level by O(ulp / (kappa dt)), and K0 and K1 V + K2 V' nearly cancel,
+// so their separate roundings add a drift of O(ulp K0) per step. kb is O(dt) although K1 and K2
+// are O(1/sigma): it is formed in FP64 and rounded once, so no FP32 product of size 1/sigma
+// cancels in the step.
 //
 // With the martingale correction the log-price step is written without K0 and K1:
 //   y += ma (V' - m) - mh (V + V') + (A m - ln E[exp(A V')]) log2(e) + mr + sqrt(k3 (V + V')) Z
@@ -45,13 +48,12 @@ constexpr unsigned long long kMaxPaths = (1ull << 37) - kTile;  // per request: 
 // A m - ln E[exp(A V')] is O((A a)^2), evaluated in forms without cancellation.
 struct DevSeg {
     float omd, th, c1h, c2h;
-    float g0, kc, kd, k3;
+    float g0, ka, kb, k3;
     float mA, ma, mh, mr;  // martingale-corrected step (HEXO_MARTINGALE)
     float delta_f;  // Asian weight of accumulations inside the run
     float dnext_f;  // weight of the accumulation after the run's last step (new dt), 0 if none
     uint32_t n_steps;
     uint32_t fin_begin, fin_count;
-    float ope;  // 1 + exp(-kappa delta)
     double delta;  // FP64 kernel: step width
     double dnext;  // FP64 kernel: dt after the finalisations
 };

@@ -3,7 +3,8 @@
 With s = min(u, 1 - u), t = s (1 - s) and l = log2 t, |Phi^-1(u)| = (1 - 2s) h(l) with h smooth in l
 (Giles' erfinv variable, M. Giles, "Approximating the erfinv function", GPU Computing Gems Jade
 Edition, 2011). h is fitted as
-  central: a degree-9 polynomial in x = l + 6.5 on l in [-11, -2],
+  central: a degree-9 polynomial on l in [-11, -2], fitted in x = l + 6.5 and expanded in powers
+           of l in FP64 (the kernel evaluates it in l, without the shift),
   tail:    a degree-8 polynomial in y = sqrt(-l) - yc on l in [log2(2^-33 (1 - 2^-33)), -11],
 by least squares on Chebyshev nodes, reweighted (Lawson) towards the minimax of the absolute error
 of z = (1 - 2s) h. The target is scipy's ndtri. Prints the coefficients as FP32 values and the
@@ -12,6 +13,8 @@ multiply-adds) on a sample of 32-bit words.
 
   python normal_f32_fit.py
 """
+from math import comb
+
 import numpy as np
 from scipy.special import ndtri
 
@@ -47,7 +50,12 @@ def lawson(x, h, r, deg, iters=80):
 def fit():
     l = nodes(L_C, -2.0)
     h, r = h_of_l(l)
-    central = lawson(l + X_C, h, r, DEG_C).astype(np.float32)
+    cx = lawson(l + X_C, h, r, DEG_C)
+    central = np.zeros(DEG_C + 1)
+    for k, ck in enumerate(cx):  # p(l + X_C) in powers of l
+        for i in range(k + 1):
+            central[i] += ck * comb(k, i) * X_C ** (k - i)
+    central = central.astype(np.float32)
     l_min = np.log2(2.0 ** -33 * (1.0 - 2.0 ** -33))
     v = nodes(np.sqrt(-L_C), np.sqrt(-l_min))
     yc = np.float32(0.5 * (np.sqrt(-L_C) + np.sqrt(-l_min)))
@@ -74,7 +82,7 @@ def normal_f32(k, central, tail, yc):
     s = f32_fma(j.astype(np.float32), np.float32(2.0 ** -32), np.float32(2.0 ** -33))
     t = f32_fma(-s, s, s)
     l = np.log2(t.astype(np.float64)).astype(np.float32)
-    hc = horner(central, (l.astype(np.float64) + X_C).astype(np.float32))
+    hc = horner(central, l)
     y = (np.sqrt(-l.astype(np.float64)).astype(np.float32).astype(np.float64) - yc).astype(np.float32)
     ht = horner(tail, y)
     h = np.where(l < np.float32(L_C), ht, hc)
@@ -92,7 +100,7 @@ def main():
     e = np.abs(z - zt)
     big = np.abs(zt) >= 1.0
     print(f"max abs error {e.max():.3e}; max relative error for |z| >= 1: {(e[big] / np.abs(zt[big])).max():.3e}")
-    print("central (x = l + 6.5), c0..c9:")
+    print("central (powers of l), c0..c9:")
     for c in central:
         print(f"    {float(c)!r}f,")
     print(f"tail (y = sqrt(-l) - {yc!r}), d0..d8:")

@@ -194,14 +194,14 @@ double orc_normal_f64_from_word(uint32_t k) {
  * libm's log2f and sqrtf where the kernel uses the hardware's: z = Phi^-1(u), u = (k + 1/2) 2^-32.
  * The word is folded onto the lower half, j = min(k, ~k), so s = (j + 1/2) 2^-32 = min(u, 1 - u)
  * keeps full relative precision in both tails; with t = s (1 - s) and l = log2 t, |z| = (1 - 2s) h(l)
- * where h is a degree-9 polynomial in l + 6.5 for l >= -11 and a degree-8 polynomial in
+ * where h is a degree-9 polynomial in l for l >= -11 and a degree-8 polynomial in
  * sqrt(-l) - 4.5305938720703125 below (Giles' erfinv variable, M. Giles, "Approximating the erfinv
  * function", GPU Computing Gems Jade Edition, 2011, in base 2). The coefficients are the product's
  * own least-squares fits; tests/test_oracle_normal_f32.py pins this function against the FP64
  * PPND16 and scipy's ndtri. */
-static const float NF32_C[10] = {2.3366379737854004f,     -0.23548607528209686f,   -0.004091143608093262f,
-                                 0.00035097255022265017f, 6.123504863353446e-05f,  3.1161430342763197e-06f,
-                                 -2.392110900473199e-07f, -4.523901608877168e-08f, -7.092287312993051e-10f,
+static const float NF32_C[10] = {0.8369221091270447f,     -0.18630073964595795f,   0.011675846762955189f,
+                                 1.11724120870349e-05f,   -0.0002003998088184744f, -9.236554433300626e-06f,
+                                 1.7887376770886476e-06f, 2.4262888587145426e-07f, 1.1781071229677309e-08f,
                                  2.1350940460074241e-10f};
 static const float NF32_D[9] = {4.8360700607299805f,     1.2490957975387573f,      -0.011836512014269829f,
                                 0.0019035495351999998f,  -0.00013401612523011863f, -0.00016207653970923275f,
@@ -219,7 +219,7 @@ float orc_normal_f32_from_word(uint32_t k) {
     const float t = fmaf(-s, s, s);
     const float l = log2f(t);
     const float h = (l < -11.0f) ? horner_fmaf(NF32_D, 9, sqrtf(-l) - 4.5305938720703125f)
-                                 : horner_fmaf(NF32_C, 10, l + 6.5f);
+                                 : horner_fmaf(NF32_C, 10, l);
     const float pr = h * fmaf(-2.0f, s, 1.0f);
     return (k & 0x80000000u) ? pr : -pr;
 }

@@ -10,7 +10,8 @@ restatement of device.hpp normal_word_f32), pinned by what the mathematics fixes
 - monotonicity in k, up to the 2e-6 step allowed where the two polynomials meet.
 
 The same check rejects CPU copies with a planted mistake (a dropped top coefficient, a wrong sign
-of one coefficient, a missing shift of the central variable, the tail switch at the wrong place).
+of one coefficient, the fit's shifted variable l + 6.5 used with the expanded coefficients, the
+tail switch at the wrong place).
 """
 import os
 import sys
@@ -85,14 +86,14 @@ def test_fit_script_reproduces_the_coefficients():
 def _planted(kind):
     central, tail, yc = F.fit()
     central, tail = central.copy(), tail.copy()
-    shift = F.X_C
+    shift = 0.0
     switch = F.L_C
     if kind == "drop_top":
         central[-1] = 0.0
     elif kind == "coef_sign":
         central[3] = -central[3]
-    elif kind == "no_shift":
-        shift = 0.0
+    elif kind == "shifted":
+        shift = F.X_C
     elif kind == "switch":
         switch = -9.0
 
@@ -115,7 +116,7 @@ def test_check_accepts_the_unplanted_copy():
     assert ok, msg
 
 
-@pytest.mark.parametrize("kind", ["drop_top", "coef_sign", "no_shift", "switch"])
+@pytest.mark.parametrize("kind", ["drop_top", "coef_sign", "shifted", "switch"])
 def test_check_rejects_planted_mistakes(kind):
     ok, _ = check_normal_f32(_planted(kind))
     assert not ok, kind
