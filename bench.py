@@ -34,12 +34,13 @@ PEAK_FP64_TFLOPS = 78.6
 
 # Algorithmic FP32 flops of the path kernel (kernels.hip), convention: add/sub/mul = 1, fma = 2,
 # exp2/log2/sqrt/rcp = 1; compares, selects, conversions and integer (RNG) work are not counted.
-F_STEP = 24      # moments th - V, m, s2, m^2, 1.5 m^2 (7); V', V' - V, V + V' - 2 th and the log-price
-                 # step incl. exp2 (16); Asian sum (1)
-F_QUAD = 17      # quadratic variance branch without its normal: 2/psi, b^2, b, a, V' - m = a (Z (Z + 2b) - 1)
-F_EXP = 21       # exponential variance branch incl. its uniform (two Newton-refined reciprocals), V' - m
-F_CENTRAL = 27   # one normal, central polynomial (s, t, log2, x, 9 FMA Horner, 1 - 2s, product)
-F_TAIL = 26      # one normal, tail polynomial (s, t, log2, sqrt, y, 8 FMA Horner, 1 - 2s, product)
+F_STEP = 21      # moments th - V, m, s2/2, m^2, rcp, m^2/(s2/2) (8); V', V + V', the log-price increment
+                 # g0 + ka (V' - m) + kb (th - V), k3 (V + V'), sqrt, y update, exp2 (12); Asian sum (1)
+F_QUAD = 14      # quadratic variance branch without its normal: tt - 1, sqrt(tt (tt - 1)), b^2, b, 1 + b^2,
+                 # a, V' - m = a (Z (Z + 2b) - 1)
+F_EXP = 22       # exponential variance branch incl. its uniform (two Newton-refined reciprocals), V' - m
+F_CENTRAL = 26   # one normal, central polynomial (s, t, log2, 9-FMA Horner, 1 - 2s, product)
+F_TAIL = 26      # one normal, tail polynomial (s, t, log2, sqrt, y, 8-FMA Horner, 1 - 2s, product)
 F_SEG = 8        # per path and run of steps: Asian run sum folding and the new-dt accumulation
 F_FIN = 6        # per path and expiry: X_T interpolation and the Asian average
 F_PAYOFF = 1     # per path and output (FP32 subtraction; the sums are FP64)

@@ -403,7 +403,11 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
             // p = (psi-1)/(psi+1), beta = 2/(m (psi+1)); V' = ln((1-p)/(1-U))/beta if U > p, else 0.
             // With sm = s2 + m^2: 1-p = 2m^2/sm and 1/beta = sm/(2m).
             const float sm = __builtin_fmaf(2.0f, s2h[j], m2[j]);
-            const float lhs = one_minus_uniform_f32(wv[j]) * sm;  // (1-U) sm
+            // (1-U) sm, 1 - U from ~k; the complement is taken here (asm: not hoisted in front of the
+            // branch, where every lane would pay for it)
+            uint32_t nk;
+            asm("v_not_b32 %0, %1" : "=v"(nk) : "v"(wv[j]));
+            const float lhs = uniform_f32(nk) * sm;
             const float tm2 = m2[j] + m2[j];
             const float lg = __builtin_amdgcn_logf(tm2 * rcp_refined(lhs));
             const float ib = sm * rcp_refined(m[j]);  // 2 / beta
@@ -482,7 +486,10 @@ __device__ __forceinline__ void qe_step_fp32(float (&V)[PPT], float (&Y)[PPT], f
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         bool tv, tx;
-        zv[j] = normal_word_f32(wv[j], quad[j], nv, tv);
+        // Z_V with its tail polynomial wherever the word needs it, also on the rare exponential-
+        // branch lanes that do not read it: the branch on the quadratic test then comes once, at
+        // the variance update, instead of also in front of the tail polynomial
+        zv[j] = normal_word_f32(wv[j], true, nv, tv);
         zx[j] = normal_word_f32(wx[j], true, nv, tx);
         if constexpr (COUNT) count_step(cnt, valid[j], quad[j], tv, tx);
     }
