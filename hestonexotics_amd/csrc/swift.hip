@@ -14,6 +14,7 @@
 // characteristic-function expression evaluated in forward-mode dual arithmetic carrying all five
 // parameter directions, instead of hand-derived partials (src/HDistribution.cpp:47-84).
 #include "swift.hpp"
+#include "swift_grid.hpp"
 #include "cplx_device.hpp"
 
 #include <hip/hip_runtime.h>
@@ -407,6 +408,39 @@ __global__ __launch_bounds__(256) void k_swift_sum(SwiftArgs a, uint64_t total) 
         a.grads[o * 5 + (k - 1)] = v;
 }
 
+// The dynamic grid of chain blockIdx.x (SwiftParameters ctor, src/SWIFT.cpp:21-35), one wavefront
+// per chain. The reference loops m = 0, 1, ... while int_error(m) > trunc and keeps one past the
+// first m that stops it (src/HDistribution.cpp:8-11: |chf(2^m pi) + chf(-2^m pi)| / (4 2^m pi^2 T));
+// here lane m evaluates scale m for m = 0..30 at once and the lowest lane whose error does not
+// exceed trunc (a NaN error stops the loop too) is that first m. Lanes past it evaluate scales the
+// loop never reaches; their values are not used.
+__global__ __launch_bounds__(64) void k_swift_grid(const SwiftGridIn* in, SwiftGridOut* out, uint32_t n) {
+    const uint32_t c = blockIdx.x;
+    if (c >= n) return;
+    const int lane = (int)threadIdx.x;
+    const SwiftGridIn& q = in[c];
+    bool stop = false;
+    if (lane <= 30) {
+        const double e2m = ::ldexp(1.0, lane);
+        const double u = e2m * kPi;
+        const Cplx a = chf_expr<Cplx>(cx(q.p[0]), cx(q.p[1]), cx(q.p[2]), cx(q.p[3]), cx(q.p[4]), q.T, u);
+        const Cplx b = chf_expr<Cplx>(cx(q.p[0]), cx(q.p[1]), cx(q.p[2]), cx(q.p[3]), cx(q.p[4]), q.T, -u);
+        const double err = ::hypot(a.re + b.re, a.im + b.im) / (4.0 * e2m * kPi * kPi * q.T);
+        stop = !(err > q.trunc);
+    }
+    const uint64_t mask = __builtin_amdgcn_ballot_w64(stop);
+    if (lane == 0) {
+        SwiftGridOut o{};
+        if (mask == 0) {
+            o.status = kSwiftGridNoScale;
+        } else {
+            const uint32_t m = (uint32_t)__builtin_ctzll(mask) + 1u;
+            o.status = swift_grid_finish(q, m, o.g, o.J_wanted);
+        }
+        out[c] = o;
+    }
+}
+
 inline dim3 grid_of(uint64_t n, uint32_t block) { return dim3((uint32_t)((n + block - 1) / block)); }
 
 }  // namespace
@@ -428,6 +462,12 @@ hipError_t launch_swift(const SwiftArgs& a, const SwiftTotals& t, hipStream_t st
     if (t.dcoef) k_swift_coeff<<<grid_of(t.dcoef * 64u, 256), 256, 0, st>>>(a, t.dcoef);
     k_swift_chf<<<grid_of(t.j, 256), 256, 0, st>>>(a, t.j);
     k_swift_sum<<<(uint32_t)t.sblocks, 256, 0, st>>>(a, t.sblocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_swift_grid(const SwiftGridIn* in, SwiftGridOut* out, uint32_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_swift_grid<<<n, 64, 0, st>>>(in, out, n);
     return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
-// swift_host.cpp -- host side of the SWIFT European pricer: the dynamic grid (the reference's
-// SwiftParameters constructor, src/SWIFT.cpp:21-35), validation, packing of one call's chains
-// into one device region, and the C entry points hexo_swift_grid_for / hexo_swift_price.
+// swift_host.cpp -- host side of the SWIFT European pricer: validation, the dynamic grid (the
+// reference's SwiftParameters constructor, src/SWIFT.cpp:21-35) for hexo_swift_grid_for (a call's
+// grids are chosen on the GPU, swift_grid.hpp), packing of one call's chains into one device
+// region, and the C entry points hexo_swift_grid_for / hexo_swift_price.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -8,12 +9,12 @@
 #include <complex>
 #include <cstring>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "../../include/hexo.h"
 #include "engine.hpp"
 #include "swift.hpp"
+#include "swift_grid.hpp"
 
 namespace {
 
@@ -37,46 +38,6 @@ cd heston_chf(const double p[5], double tau, double u) {
 double int_error(const double p[5], double tau, unsigned m) {
     const double e2m = std::exp2((double)m);
     return std::abs(heston_chf(p, tau, e2m * kPi) + heston_chf(p, tau, -e2m * kPi)) / (4.0 * e2m * kPi * kPi * tau);
-}
-
-// Cumulants that size the truncation range (HDistribution::first/second/fourth_order_moment,
-// src/HDistribution.cpp:86-113), in the model's parameter names: p = (v0, theta, rho, kappa, sigma).
-double cumulant1(const double p[5], double t) { return -0.5 * p[1] * t; }
-double cumulant2(const double p[5], double t) {
-    const double theta = p[1], rho = p[2], kappa = p[3], sigma = p[4];
-    const double kappa2 = kappa * kappa, kappa3 = kappa2 * kappa, sigma2 = sigma * sigma;
-    return theta / (8 * kappa3) *
-           (-sigma2 * std::exp(-2 * kappa * t) + 4 * sigma * std::exp(-kappa * t) * (sigma - 2 * kappa * rho) +
-            2 * kappa * t * (4 * kappa2 + sigma2 - 4 * kappa * sigma * rho) + sigma * (8 * kappa * rho - 3 * sigma));
-}
-double cumulant4(const double p[5], double t) {
-    const double theta = p[1], rho = p[2], kappa = p[3], sigma = p[4];
-    const double kappa2 = kappa * kappa, kappa3 = kappa2 * kappa, kappa4 = kappa3 * kappa;
-    const double sigma2 = sigma * sigma, sigma3 = sigma2 * sigma, sigma4 = sigma3 * sigma;
-    const double t2 = t * t, rho2 = rho * rho, q = sigma - 2 * kappa * rho;
-    const double e1 = std::exp(-kappa * t), e2 = std::exp(-2 * kappa * t), e3 = std::exp(-3 * kappa * t),
-                 e4 = std::exp(-4 * kappa * t);
-    const double b4 = -3 * sigma4 * e4;
-    const double b3 = -8 * sigma2 * e3 * (2 * kappa * sigma * t * q + 4 * kappa2 + sigma2 - 6 * kappa * sigma * rho);
-    const double b2 =
-        -4 * e2 *
-        (4 * kappa2 * sigma2 * t2 * q * q +
-         2 * kappa * sigma * t *
-             (sigma3 - 16 * kappa3 * rho - 12 * kappa * sigma2 * rho + 4 * kappa2 * sigma * (3 + 4 * rho2)) +
-         8 * kappa4 - 3 * sigma4 - 32 * kappa3 * sigma * rho + 8 * kappa * sigma3 * rho + 16 * kappa2 * sigma2 * rho2);
-    const double b1 = -8 * e1 *
-                      (-2 * kappa2 * sigma * t2 * q * q * q -
-                       8 * kappa * t *
-                           (sigma4 - 7 * kappa * sigma3 * rho + 4 * kappa4 * rho2 -
-                            8 * kappa3 * sigma * rho * (1 + rho2) + kappa2 * sigma2 * (3 + 14 * rho2)) -
-                       9 * sigma4 + 70 * kappa * sigma3 * rho + 32 * kappa3 * sigma * rho * (4 + 3 * rho2) -
-                       16 * kappa4 * (1 + 4 * rho2) - 4 * kappa2 * sigma2 * (9 + 40 * rho2));
-    const double b0 = 4 * kappa * t *
-                          (5 * sigma4 - 40 * kappa * sigma3 * rho - 32 * kappa3 * sigma * rho * (3 + 2 * rho2) +
-                           16 * kappa4 * (1 + 4 * rho2) + 24 * kappa2 * sigma2 * (1 + 4 * rho2)) -
-                      73 * sigma4 + 544 * kappa * sigma3 * rho + 128 * kappa3 * sigma * rho * (7 + 6 * rho2) -
-                      32 * kappa4 * (3 + 16 * rho2) - 64 * kappa2 * sigma2 * (4 + 19 * rho2);
-    return (3 * sigma2 * theta) / (64 * std::pow(kappa, 7)) * (b4 + b3 + b2 + b1 + b0);
 }
 
 bool finite_pos(double x) { return std::isfinite(x) && x > 0.0; }
@@ -131,45 +92,52 @@ int check_grid(const hexo_swift_grid& g, std::string& err) {
 }
 
 // SwiftParameters constructor (src/SWIFT.cpp:21-35).
+// The extreme strikes and the rest of a chain's grid-choice inputs.
+hexo::SwiftGridIn grid_input(const hexo_swift_request& q) {
+    hexo::SwiftGridIn in{{q.v0, q.theta, q.rho, q.kappa, q.sigma}, q.T, q.r, q.S0, q.strikes[0], q.strikes[0], q.trunc};
+    for (uint32_t i = 1; i < q.n_strikes; ++i) {
+        in.kmin = std::min(in.kmin, q.strikes[i]);
+        in.kmax = std::max(in.kmax, q.strikes[i]);
+    }
+    return in;
+}
+
+int grid_status(int32_t status, double J_wanted, std::string& err) {
+    switch (status) {
+        case hexo::kSwiftGridOk: return HEXO_OK;
+        case hexo::kSwiftGridNoScale: err = "no wavelet scale reaches the truncation precision"; break;
+        case hexo::kSwiftGridOverflow: err = "SWIFT index range overflows"; break;
+        default: err = "SWIFT grid needs J = " + std::to_string(J_wanted) + " (outside [1, HEXO_SWIFT_MAX_J])"; break;
+    }
+    return HEXO_ERR_INVALID;
+}
+
+// SwiftParameters constructor (src/SWIFT.cpp:21-35) on the host.
 int dynamic_grid(const hexo_swift_request& q, hexo_swift_grid& g, std::string& err) {
-    const double p[5] = {q.v0, q.theta, q.rho, q.kappa, q.sigma};
+    const hexo::SwiftGridIn in = grid_input(q);
     unsigned m = 0;
     // the reference loops `while (int_error(m++) > trunc)`: m ends one past the first scale
     // whose error bound is below trunc
-    while (int_error(p, q.T, m++) > q.trunc)
-        if (m > 30) {
-            err = "no wavelet scale reaches the truncation precision";
-            return HEXO_ERR_INVALID;
-        }
-    double kmin = q.strikes[0], kmax = q.strikes[0];
-    for (uint32_t i = 1; i < q.n_strikes; ++i) {
-        kmin = std::min(kmin, q.strikes[i]);
-        kmax = std::max(kmax, q.strikes[i]);
+    while (int_error(in.p, q.T, m++) > q.trunc)
+        if (m > 30) return grid_status(hexo::kSwiftGridNoScale, 0.0, err);
+    double J = 0.0;
+    return grid_status(hexo::swift_grid_finish(in, m, g, J), J, err);
+}
+
+// Grows the context's SWIFT device workspace to at least `bytes` (kept for the context's lifetime).
+int swift_workspace(hexo_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->swift_dev_bytes) return HEXO_OK;
+    // the previous workspace may still be read by work queued on the stream
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->set_error(HEXO_ERR_HIP, "stream sync failed");
+    if (ctx->swift_dev) (void)hipFree(ctx->swift_dev);
+    ctx->swift_dev = nullptr;
+    ctx->swift_dev_bytes = 0;
+    const size_t want = bytes + bytes / 2;
+    if (hipMalloc(&ctx->swift_dev, want) != hipSuccess) {
+        ctx->swift_dev = nullptr;
+        return ctx->set_error(HEXO_ERR_NOMEM, "hipMalloc failed");
     }
-    const double hi = q.r * q.T + std::log(q.S0 / kmin);
-    const double lo = q.r * q.T + std::log(q.S0 / kmax);
-    const double c = std::fabs(cumulant1(p, q.T)) +
-                     10.0 * std::sqrt(std::fabs(cumulant2(p, q.T)) + std::sqrt(std::fabs(cumulant4(p, q.T))));
-    g.m = m;
-    g.exp2_m = 1u << m;
-    g.sqrt_exp2_m = std::sqrt((double)g.exp2_m);
-    g.lower = lo - c;
-    g.upper = hi + c;
-    const double k1 = std::ceil(g.exp2_m * g.lower), k2 = std::floor(g.exp2_m * g.upper);
-    if (!(std::fabs(k1) < 2e9 && std::fabs(k2) < 2e9)) {
-        err = "SWIFT index range overflows";
-        return HEXO_ERR_INVALID;
-    }
-    g.k1 = (int32_t)k1;
-    g.k2 = (int32_t)k2;
-    const double iota = std::ceil(std::log2(kPi * std::fabs(k1 - k2))) - 1.0;
-    const double J = std::exp2(iota - 1.0);
-    if (!(J >= 1.0 && J <= (double)HEXO_SWIFT_MAX_J)) {
-        err = "SWIFT grid needs J = " + std::to_string(J) + " (outside [1, HEXO_SWIFT_MAX_J])";
-        return HEXO_ERR_INVALID;
-    }
-    g.J = (uint32_t)J;
-    g.reserved = 0;
+    ctx->swift_dev_bytes = want;
     return HEXO_OK;
 }
 
@@ -197,22 +165,51 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
     if (!reqs || !prices || n == 0) return ctx->set_error(HEXO_ERR_INVALID, "requests and prices must be non-NULL");
     std::string err;
     std::vector<hexo_swift_grid> grids(n);
-    {
-        // the dynamic grids (a few characteristic-function evaluations each) on host threads
-        std::vector<int> rcs(n, HEXO_OK);
-        std::vector<std::string> errs(n);
-        const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
-        const uint32_t n_thr = std::min<uint32_t>(std::min<uint32_t>(hw, 16u), std::max<uint32_t>(1u, n / 32u));
-        auto work = [&](uint32_t t) {
-            for (uint32_t i = t; i < n; i += n_thr) rcs[i] = grid_of(reqs[i], grids[i], errs[i]);
-        };
-        std::vector<std::thread> pool;
-        for (uint32_t t = 1; t < n_thr; ++t) pool.emplace_back(work, t);
-        work(0);
-        for (auto& th : pool) th.join();
-        for (uint32_t i = 0; i < n; ++i)
-            if (rcs[i]) return ctx->set_error(rcs[i], "swift request " + std::to_string(i) + ": " + errs[i]);
+    std::vector<uint32_t> dyn;  // chains whose grid is chosen here (not HEXO_SWIFT_FIXED_GRID)
+    for (uint32_t i = 0; i < n; ++i) {
+        const int rc = check_request(reqs[i], err);
+        if (rc) return ctx->set_error(rc, "swift request " + std::to_string(i) + ": " + err);
+        if (reqs[i].flags & HEXO_SWIFT_FIXED_GRID)
+            grids[i] = reqs[i].grid;
+        else
+            dyn.push_back(i);
     }
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    if (hipSetDevice(ctx->device) != hipSuccess) return ctx->set_error(HEXO_ERR_HIP, "hipSetDevice failed");
+    auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    if (!dyn.empty()) {
+        // the dynamic grids of all chains in one launch (a wavefront per chain), through the
+        // call's pinned stage and device workspace, which the packing below then reuses
+        const size_t nd = dyn.size();
+        const size_t b_in = align(sizeof(hexo::SwiftGridIn) * nd), b_out = sizeof(hexo::SwiftGridOut) * nd;
+        if (!ctx->swift_host.reserve(b_in + b_out)) return ctx->set_error(HEXO_ERR_NOMEM, "hipHostMalloc failed");
+        if (int rc = swift_workspace(ctx, b_in + b_out)) return rc;
+        unsigned char* host = reinterpret_cast<unsigned char*>(ctx->swift_host.p);
+        unsigned char* dev = reinterpret_cast<unsigned char*>(ctx->swift_dev);
+        hexo::SwiftGridIn* gin = reinterpret_cast<hexo::SwiftGridIn*>(host);
+        for (size_t k = 0; k < nd; ++k) gin[k] = grid_input(reqs[dyn[k]]);
+        hipStream_t st = ctx->stream;
+        const bool ok =
+            hipMemcpyAsync(dev, host, sizeof(hexo::SwiftGridIn) * nd, hipMemcpyHostToDevice, st) == hipSuccess &&
+            hexo::launch_swift_grid(reinterpret_cast<const hexo::SwiftGridIn*>(dev),
+                                    reinterpret_cast<hexo::SwiftGridOut*>(dev + b_in), (uint32_t)nd, st) ==
+                hipSuccess &&
+            hipMemcpyAsync(host + b_in, dev + b_in, b_out, hipMemcpyDeviceToHost, st) == hipSuccess &&
+            hipStreamSynchronize(st) == hipSuccess;
+        if (!ok) {
+            (void)hipStreamSynchronize(st);
+            return ctx->set_error(HEXO_ERR_HIP, "SWIFT grid launch failed");
+        }
+        const hexo::SwiftGridOut* gout = reinterpret_cast<const hexo::SwiftGridOut*>(host + b_in);
+        for (size_t k = 0; k < nd; ++k) {
+            if (int rc = grid_status(gout[k].status, gout[k].J_wanted, err))
+                return ctx->set_error(rc, "swift request " + std::to_string(dyn[k]) + ": " + err);
+            grids[dyn[k]] = gout[k].g;
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (int rc = check_grid(grids[i], err))
+            return ctx->set_error(rc, "swift request " + std::to_string(i) + ": " + err);
     // host image of the device region: records, tables, FFT chain list, prefix sums, strikes;
     // then the work arrays
     std::vector<hexo::DevSwift> recs(n);
@@ -284,7 +281,6 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
     std::vector<uint64_t> tab_end(tables.size());
     for (size_t t = 0; t < tables.size(); ++t) tab_end[t] = tables[t].off + 4ull * tables[t].J;
     const uint32_t n_tab = (uint32_t)tables.size();
-    auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t b_recs = align(sizeof(hexo::DevSwift) * n), b_tabs = align(sizeof(hexo::SwiftTable) * n_tab),
                  b_tabend = align(8 * (size_t)n_tab), b_fft = align(4 * fft_chains.size()),
                  b_ends = align(sizeof(uint64_t) * 4 * (size_t)n), b_strk = align(8 * n_strk),
@@ -294,23 +290,9 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
     const size_t in_bytes = b_recs + b_tabs + b_tabend + b_fft + b_ends + b_strk;
     const size_t total = in_bytes + b_tw + b_pay + b_dens + b_coeff + b_pm + b_prices + b_grads;
     const size_t out_bytes = 8 * n_strk + (grads ? 40 * n_strk : 0);
-    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
-    if (hipSetDevice(ctx->device) != hipSuccess) return ctx->set_error(HEXO_ERR_HIP, "hipSetDevice failed");
     if (!ctx->swift_host.reserve(std::max(in_bytes, out_bytes)))
         return ctx->set_error(HEXO_ERR_NOMEM, "hipHostMalloc failed");
-    if (total > ctx->swift_dev_bytes) {
-        // the previous workspace may still be read by work queued on the stream
-        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->set_error(HEXO_ERR_HIP, "stream sync failed");
-        if (ctx->swift_dev) (void)hipFree(ctx->swift_dev);
-        ctx->swift_dev = nullptr;
-        ctx->swift_dev_bytes = 0;
-        const size_t want = total + total / 2;
-        if (hipMalloc(&ctx->swift_dev, want) != hipSuccess) {
-            ctx->swift_dev = nullptr;
-            return ctx->set_error(HEXO_ERR_NOMEM, "hipMalloc failed");
-        }
-        ctx->swift_dev_bytes = want;
-    }
+    if (int rc = swift_workspace(ctx, total)) return rc;
     unsigned char* host = reinterpret_cast<unsigned char*>(ctx->swift_host.p);
     std::memset(host, 0, in_bytes);
     size_t off = 0;

@@ -66,6 +66,37 @@ def test_swift_matches_oracle_on_dynamic_grids(ctx, oracle):
         np.testing.assert_allclose(g, og, rtol=1e-9, atol=1e-10 * ch.S0)
 
 
+def test_gpu_grid_choice_equals_host(ctx):
+    """The grids hexo_swift_price chooses on the GPU (one wavefront per chain, every scale tried at
+    once) equal hexo_swift_grid_for's host loop over the calibration batch's 512 chains and a wide
+    parameter box: the integer parameters (m, 2^m, k1, k2, J) exactly; the range ends to 1e-10
+    relative. The fourth cumulant's bracket cancels terms ~1e4 times its value at small kappa T,
+    so the device's and the host's libm (both within an ulp) leave the ends up to 2e-11 apart
+    (measured); the prices move by the same relative amount."""
+    say("gpu swift grid choice == host")
+    import bench
+    rng = np.random.default_rng(17)
+    chains = bench.swift_chains(64)
+    for _ in range(200):
+        v0, th = rng.uniform(0.005, 0.5, 2)
+        rho, kappa, sigma = rng.uniform(-0.99, 0.99), rng.uniform(0.05, 20.0), rng.uniform(0.05, 2.0)
+        T = float(np.exp(rng.uniform(np.log(0.02), np.log(10.0))))
+        K = sorted(rng.uniform(40.0, 250.0, int(rng.integers(1, 40))))
+        chains.append(H.SwiftChain(v0, th, rho, kappa, sigma, S0=100.0, T=T, strikes=list(K),
+                                   r=float(rng.uniform(0.0, 0.08)), trunc=float(10.0 ** rng.uniform(-12, -6))))
+    ok = []
+    for ch in chains:
+        try:
+            ok.append((ch, H.swift_grid(ch)))
+        except H.HexoError:
+            pass  # grids the host refuses (J beyond HEXO_SWIFT_MAX_J) are refused by the call too
+    assert len(ok) > 600
+    res = ctx.swift_price([c for c, _ in ok])
+    for (ch, host), (_, _, used) in zip(ok, res):
+        assert [used[i] for i in (0, 1, 5, 6, 7)] == [host[i] for i in (0, 1, 5, 6, 7)], (ch, used, host)
+        assert np.allclose(used, host, rtol=1e-10, atol=0)
+
+
 def test_swift_batch_equals_single(ctx):
     say("gpu swift batch == single")
     chains = dynamic_chains()
