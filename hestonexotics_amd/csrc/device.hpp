@@ -116,37 +116,21 @@ __device__ __forceinline__ double normal_f64(uint32_t k) {
     return (q < 0.0) ? -z : z;
 }
 
-// a * K + b (v_fmamk_f32) and a * b + K (v_fmaak_f32) with K a 32-bit literal and the other operands
-// in vector registers: the VOP2 forms that encode a literal. A VOP3 v_fma_f32 cannot, so for a
-// constant operand the compiler reads a scalar register instead, and on gfx950 an FP32 instruction
-// with a scalar-register operand issues at about half the rate of one without (measured). The
-// result is the same fused multiply-add, bit for bit. The compiler does not insert wait states in
-// front of inline asm: an operand must not be the result of a transcendental (v_log, v_exp,
-// v_sqrt, v_rcp, ...) issued just before, which needs one.
-template <uint32_t K>
-__device__ __forceinline__ float fmamk_lit(float a, float b) {
-    float d;
-    asm("v_fmamk_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "i"(K), "v"(b));
-    return d;
-}
-template <uint32_t K>
-__device__ __forceinline__ float fmaak_lit(float a, float b) {
-    float d;
-    asm("v_fmaak_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "i"(K));
-    return d;
-}
-#define HEXO_F32_BITS(x) __builtin_bit_cast(uint32_t, (float)(x))
-
 // A value the compiler must keep in a vector register (it cannot prove the result wave-uniform and
-// fold it back into a scalar operand).
+// fold it back into a scalar operand): on gfx950 an FP32 or bitwise instruction with a
+// scalar-register operand issues at about half the rate of one without (measured). The compiler
+// does not insert wait states in front of inline asm, and a transcendental (v_log, v_exp, v_sqrt,
+// v_rcp, ...) needs one before the next instruction reads its result or overwrites its source
+// register, whichever instruction the scheduler puts before the asm; so the asm opens with its own
+// wait state (s_nop 0).
 __device__ __forceinline__ float in_vgpr(float x) {
     float d;
-    asm("v_mov_b32 %0, %1" : "=v"(d) : "v"(x));
+    asm("s_nop 0\n\tv_mov_b32 %0, %1" : "=v"(d) : "v"(x));
     return d;
 }
 __device__ __forceinline__ uint32_t in_vgpr_u32(uint32_t x) {
     uint32_t d;
-    asm("v_mov_b32 %0, %1" : "=v"(d) : "v"(x));
+    asm("s_nop 0\n\tv_mov_b32 %0, %1" : "=v"(d) : "v"(x));
     return d;
 }
 
@@ -175,8 +159,9 @@ struct NormalF32 {
                                    0.00019050169794354588f,  -0.00012945415801368654f, 3.693158214446157e-05f};
 };
 
-// The constants of the evaluation that the literal-form instructions cannot encode (the second
-// operand of a first Horner step, 2^-33, the sign bit), made once per thread in vector registers.
+// The constants of the evaluation that the literal forms cannot carry (the second operand of the
+// first Horner step, 2^-33, the sign mask of a three-input bitwise op), made once per thread in
+// vector registers.
 struct NormalVregs {
     float c8, h33;
     uint32_t sign;
@@ -190,32 +175,31 @@ __device__ __forceinline__ NormalVregs normal_vregs() {
 // polynomial's value, meaningless in the tail).
 __device__ __forceinline__ float normal_word_f32(uint32_t k, bool need, const NormalVregs& nv, bool& tail) {
     const uint32_t j = k ^ (uint32_t)((int32_t)k >> 31);
-    const float s = fmamk_lit<HEXO_F32_BITS(0x1p-32f)>((float)j, nv.h33);
+    const float s = __builtin_fmaf((float)j, 0x1p-32f, nv.h33);
     const float t = __builtin_fmaf(-s, s, s);
     const float l = __builtin_amdgcn_logf(t);
-    // The first Horner step is left to the compiler (it forms v_fmamk_f32 itself here): it reads
-    // the log2's result, and only the compiler inserts the wait state that a transcendental's
-    // result needs before the next instruction reads it; inline asm gets none.
+    // Horner steps the compiler forms as v_fmamk_f32 / v_fmaak_f32 with the coefficient as the
+    // instruction's literal (no scalar-register operand)
     float h = __builtin_fmaf(l, NormalF32::c[9], nv.c8);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[7])>(h, l);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[6])>(h, l);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[5])>(h, l);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[4])>(h, l);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[3])>(h, l);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[2])>(h, l);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[1])>(h, l);
-    h = fmaak_lit<HEXO_F32_BITS(NormalF32::c[0])>(h, l);
+    h = __builtin_fmaf(h, l, NormalF32::c[7]);
+    h = __builtin_fmaf(h, l, NormalF32::c[6]);
+    h = __builtin_fmaf(h, l, NormalF32::c[5]);
+    h = __builtin_fmaf(h, l, NormalF32::c[4]);
+    h = __builtin_fmaf(h, l, NormalF32::c[3]);
+    h = __builtin_fmaf(h, l, NormalF32::c[2]);
+    h = __builtin_fmaf(h, l, NormalF32::c[1]);
+    h = __builtin_fmaf(h, l, NormalF32::c[0]);
     tail = l < NormalF32::lc;
     if (need && tail) {
         const float y = __builtin_amdgcn_sqrtf(-l) - NormalF32::yc;
-        float g = fmamk_lit<HEXO_F32_BITS(NormalF32::d[8])>(y, in_vgpr(NormalF32::d[7]));
-        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[6])>(g, y);
-        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[5])>(g, y);
-        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[4])>(g, y);
-        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[3])>(g, y);
-        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[2])>(g, y);
-        g = fmaak_lit<HEXO_F32_BITS(NormalF32::d[1])>(g, y);
-        h = fmaak_lit<HEXO_F32_BITS(NormalF32::d[0])>(g, y);
+        float g = __builtin_fmaf(y, NormalF32::d[8], NormalF32::d[7]);
+        g = __builtin_fmaf(g, y, NormalF32::d[6]);
+        g = __builtin_fmaf(g, y, NormalF32::d[5]);
+        g = __builtin_fmaf(g, y, NormalF32::d[4]);
+        g = __builtin_fmaf(g, y, NormalF32::d[3]);
+        g = __builtin_fmaf(g, y, NormalF32::d[2]);
+        g = __builtin_fmaf(g, y, NormalF32::d[1]);
+        h = __builtin_fmaf(g, y, NormalF32::d[0]);
     }
     const float pr = h * __builtin_fmaf(-2.0f, s, 1.0f);  // |z| = h (1 - 2s)
     // negative for k < 2^31 (u < 1/2)

@@ -326,6 +326,12 @@ __device__ __forceinline__ void count_flush(unsigned long long* counts, uint32_t
 // ------------------------------------------------------------------------------------------
 // FP32 path kernel
 // ------------------------------------------------------------------------------------------
+// No implicit FMA contraction in the FP32 path: every fused multiply-add below is written out.
+// With contraction the compiler fuses a product into a later add only where both land in one basic
+// block, which depends on how each paths-per-thread instance is laid out (one path per thread
+// flattens the quadratic branch and turned tt - 1 into fma(m^2, 1/s2h, -1)), so the paths would
+// change by an ulp with the launch shape.
+#pragma clang fp contract(off)
 struct SegF {
     float omd, th, c1h, c2h, g0, ka, kb, k3;
     float mA, ma, mh, mr;
@@ -404,9 +410,10 @@ __device__ __forceinline__ void qe_update_fp32(float (&V)[PPT], float (&Y)[PPT],
             // With sm = s2 + m^2: 1-p = 2m^2/sm and 1/beta = sm/(2m).
             const float sm = __builtin_fmaf(2.0f, s2h[j], m2[j]);
             // (1-U) sm, 1 - U from ~k; the complement is taken here (asm: not hoisted in front of the
-            // branch, where every lane would pay for it)
+            // branch, where every lane would pay for it; the wait state as at the asm helpers in
+            // device.hpp)
             uint32_t nk;
-            asm("v_not_b32 %0, %1" : "=v"(nk) : "v"(wv[j]));
+            asm("s_nop 0\n\tv_not_b32 %0, %1" : "=v"(nk) : "v"(wv[j]));
             const float lhs = uniform_f32(nk) * sm;
             const float tm2 = m2[j] + m2[j];
             const float lg = __builtin_amdgcn_logf(tm2 * rcp_refined(lhs));
@@ -691,6 +698,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PPT == 2 ? 
         path_fp32<PPT, DEBUG, COUNT, MART, GEO>(a, ri, uniform((item - first) * (uint32_t)PPT), lane, lds);
     }
 }
+#pragma clang fp contract(fast)
 // ------------------------------------------------------------------------------------------
 // Plain FP64 path kernel: the reference's step and policies in FP64, recomputing the step's
 // moments and K0..K4 every step exactly as written in src/HSimulation.tpp:52-86.
