@@ -50,7 +50,8 @@ def pmc_traffic(workload: str, paths: int, tuning: dict):
     """HBM bytes per launch of the path kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary of this workload (scaled by the copy-kernel calibration), or None when this run's
     workload or launch shape is not the one profiled."""
-    f = os.path.join(ROOT, "profiles", f"k_path_fp32_{workload}_traffic.json")
+    name = {"c3": "uneven_batch", "c2": "grid"}[workload]
+    f = os.path.join(ROOT, "profiles", f"k_path_fp32_{name}_traffic.json")
     if not os.path.exists(f):
         return None
     d = json.load(open(f))
