@@ -323,7 +323,7 @@ bool PinnedBuf::reserve(size_t bytes) {
     p = nullptr;
     cap = 0;
     const size_t want = std::max<size_t>(bytes, (size_t)1 << 20);
-    if (hipHostMalloc(reinterpret_cast<void**>(&p), want, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), want, flags) != hipSuccess) {
         p = nullptr;
         return false;
     }

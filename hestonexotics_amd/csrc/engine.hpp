@@ -91,6 +91,9 @@ class Arena {
 struct PinnedBuf {
     char* p = nullptr;
     size_t cap = 0;
+    // hipHostMallocCoherent for buffers that kernels read or write in place (fine-grained: no
+    // stale GPU-cached copy between calls)
+    unsigned flags = hipHostMallocDefault;
     bool reserve(size_t bytes);
     ~PinnedBuf();
 };
@@ -159,10 +162,15 @@ struct hexo_ctx {
     std::shared_ptr<struct hexo_server_state> server;
     std::unordered_set<struct hexo_batch*> batches;  // live batches; released by hexo_ctx_destroy
     // SWIFT pricer workspace, grown on demand and kept for the context's lifetime (no per-call
-    // allocation, which would also synchronise the device)
+    // allocation, which would also synchronise the device). The kernels read their inputs from
+    // and write their results to the two fine-grained pinned buffers in place: the grid kernel's
+    // inputs, strikes and outputs in swift_in_host; the pricing kernels' tables (which a first
+    // kernel copies to the device) and results in swift_host. No copy-engine transfer, whose
+    // hand-offs with the kernels would cost more than the transfers themselves.
     void* swift_dev = nullptr;
     size_t swift_dev_bytes = 0;
-    hexo::PinnedBuf swift_host;
+    hexo::PinnedBuf swift_host{nullptr, 0, hipHostMallocCoherent};
+    hexo::PinnedBuf swift_in_host{nullptr, 0, hipHostMallocCoherent};
     // the synchronous pricing calls' device arena and staging buffer, likewise grown on demand and
     // resident (guarded by mu): hexo_price and hexo_price_batch allocate nothing once they fit
     hexo::Arena sync_arena;

@@ -19,6 +19,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
+
 namespace hexo {
 namespace {
 
@@ -409,16 +412,35 @@ __global__ __launch_bounds__(256) void k_swift_sum(SwiftArgs a, uint64_t total) 
 }
 
 // The dynamic grid of chain blockIdx.x (SwiftParameters ctor, src/SWIFT.cpp:21-35), one wavefront
-// per chain. The reference loops m = 0, 1, ... while int_error(m) > trunc and keeps one past the
-// first m that stops it (src/HDistribution.cpp:8-11: |chf(2^m pi) + chf(-2^m pi)| / (4 2^m pi^2 T));
-// here lane m evaluates scale m for m = 0..30 at once and the lowest lane whose error does not
-// exceed trunc (a NaN error stops the loop too) is that first m. Lanes past it evaluate scales the
-// loop never reaches; their values are not used.
-__global__ __launch_bounds__(64) void k_swift_grid(const SwiftGridIn* in, SwiftGridOut* out, uint32_t n) {
+// per chain. The lanes first find the chain's extreme strikes (and whether every strike is finite
+// and positive). The reference loops m = 0, 1, ... while int_error(m) > trunc and keeps one past
+// the first m that stops it (src/HDistribution.cpp:8-11: |chf(2^m pi) + chf(-2^m pi)| /
+// (4 2^m pi^2 T)); here lane m evaluates scale m for m = 0..30 at once and the lowest lane whose
+// error does not exceed trunc (a NaN error stops the loop too) is that first m. Lanes past it
+// evaluate scales the loop never reaches; their values are not used.
+__global__ __launch_bounds__(64) void k_swift_grid(const SwiftGridIn* in, const double* strikes, SwiftGridOut* out,
+                                                   uint32_t n) {
     const uint32_t c = blockIdx.x;
     if (c >= n) return;
     const int lane = (int)threadIdx.x;
-    const SwiftGridIn& q = in[c];
+    SwiftGridIn q = in[c];
+    const double* ks = strikes + q.strike_off;
+    double lo = INFINITY, hi = -INFINITY;
+    bool bad = false;
+    for (uint32_t i = (uint32_t)lane; i < q.n_strikes; i += 64) {
+        const double k = ks[i];
+        bad |= !(k > 0.0 && k < INFINITY);
+        lo = fmin(lo, k);
+        hi = fmax(hi, k);
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        lo = fmin(lo, __shfl_xor(lo, m));
+        hi = fmax(hi, __shfl_xor(hi, m));
+    }
+    const bool any_bad = __builtin_amdgcn_ballot_w64(bad) != 0;
+    q.kmin = lo;
+    q.kmax = hi;
     bool stop = false;
     if (lane <= 30) {
         const double e2m = ::ldexp(1.0, lane);
@@ -431,13 +453,28 @@ __global__ __launch_bounds__(64) void k_swift_grid(const SwiftGridIn* in, SwiftG
     const uint64_t mask = __builtin_amdgcn_ballot_w64(stop);
     if (lane == 0) {
         SwiftGridOut o{};
-        if (mask == 0) {
+        if (any_bad) {
+            o.status = kSwiftGridBadStrike;
+        } else if (mask == 0) {
             o.status = kSwiftGridNoScale;
         } else {
             const uint32_t m = (uint32_t)__builtin_ctzll(mask) + 1u;
             o.status = swift_grid_finish(q, m, o.g, o.J_wanted);
         }
         out[c] = o;
+    }
+}
+
+// Copies two ranges of 16-byte words from fine-grained pinned host memory to the device (a call's
+// tables and its strikes), so the pricing kernels read them from HBM.
+__global__ __launch_bounds__(256) void k_swift_stage(const uint4* s0, uint4* d0, uint64_t n0, const uint4* s1,
+                                                     uint4* d1, uint64_t n1) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n0 + n1; i += stride) {
+        if (i < n0)
+            d0[i] = s0[i];
+        else
+            d1[i - n0] = s1[i - n0];
     }
 }
 
@@ -449,12 +486,22 @@ hipError_t launch_swift(const SwiftArgs& a, const SwiftTotals& t, hipStream_t st
     k_swift_twiddle<<<grid_of(t.tw, 256), 256, 0, st>>>(a, t.tw);
     k_swift_payoff<<<grid_of(t.j, 256), 256, 0, st>>>(a, t.j);
     if (t.n_fft) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_swift_fft1),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)t.fft_lds_bytes);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_swift_fft2),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)t.fft_lds_bytes);
+        // the FFT kernels' dynamic LDS limit, raised once per device to the largest FFT's need
+        static std::atomic<uint64_t> lds_set{0};  // bit d: done on device d
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
         if (e != hipSuccess) return e;
+        const uint64_t bit = 1ull << (dev & 63);
+        if (!(lds_set.load() & bit)) {
+            const int lds = (int)(sizeof(Cplx) * 2 * kSwiftFftMaxJ);
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_swift_fft1),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_swift_fft2),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+            if (e != hipSuccess) return e;
+            lds_set.fetch_or(bit);
+        }
         k_swift_fft1<<<t.n_fft, 256, t.fft_lds_bytes, st>>>(a);
     }
     if (t.dens) k_swift_dens<<<grid_of(t.dens * 64u, 256), 256, 0, st>>>(a, t.dens);
@@ -465,9 +512,21 @@ hipError_t launch_swift(const SwiftArgs& a, const SwiftTotals& t, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t launch_swift_grid(const SwiftGridIn* in, SwiftGridOut* out, uint32_t n, hipStream_t st) {
+hipError_t launch_swift_stage(const void* s0, void* d0, size_t bytes0, const void* s1, void* d1, size_t bytes1,
+                              hipStream_t st) {
+    if (bytes0 % 16 || bytes1 % 16) return hipErrorInvalidValue;
+    const uint64_t n0 = bytes0 / 16, n1 = bytes1 / 16;
+    if (n0 + n1 == 0) return hipSuccess;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n0 + n1 + 255) / 256, 1024);
+    k_swift_stage<<<blocks, 256, 0, st>>>(static_cast<const uint4*>(s0), static_cast<uint4*>(d0), n0,
+                                          static_cast<const uint4*>(s1), static_cast<uint4*>(d1), n1);
+    return hipGetLastError();
+}
+
+hipError_t launch_swift_grid(const SwiftGridIn* in, const double* strikes, SwiftGridOut* out, uint32_t n,
+                             hipStream_t st) {
     if (n == 0) return hipSuccess;
-    k_swift_grid<<<n, 64, 0, st>>>(in, out, n);
+    k_swift_grid<<<n, 64, 0, st>>>(in, strikes, out, n);
     return hipGetLastError();
 }
 

@@ -70,5 +70,9 @@ struct SwiftTotals {
 };
 
 hipError_t launch_swift(const SwiftArgs& a, const SwiftTotals& t, hipStream_t st);
+// Copies bytes0 from s0 to d0 and bytes1 from s1 to d1 (multiples of 16; the sources may be
+// fine-grained pinned host memory) in one kernel.
+hipError_t launch_swift_stage(const void* s0, void* d0, size_t bytes0, const void* s1, void* d1, size_t bytes1,
+                              hipStream_t st);
 
 }  // namespace hexo

@@ -13,10 +13,14 @@
 namespace hexo {
 
 // One chain's inputs to the grid choice: model, expiry, rate, spot, the extreme strikes and the
-// truncation precision.
+// truncation precision. The GPU grid kernel finds the extreme strikes itself, from the chain's
+// n_strikes strikes at strike_off of the call's strike array (checking them on the way).
 struct SwiftGridIn {
     double p[5];  // v0, theta, rho, kappa, sigma
     double T, r, S0, kmin, kmax, trunc;
+    uint64_t strike_off;
+    uint32_t n_strikes;
+    uint32_t pad;
 };
 
 // One chain's grid and its status: kSwiftGridOk, or which check of the choice failed.
@@ -30,6 +34,7 @@ constexpr int32_t kSwiftGridOk = 0;
 constexpr int32_t kSwiftGridNoScale = 1;   // no scale m <= 30 reaches the truncation precision
 constexpr int32_t kSwiftGridOverflow = 2;  // the wavelet index range overflows 32 bits
 constexpr int32_t kSwiftGridJRange = 3;    // J outside [1, HEXO_SWIFT_MAX_J]
+constexpr int32_t kSwiftGridBadStrike = 4; // a strike that is not finite and positive
 
 // Cumulants that size the truncation range (HDistribution::first/second/fourth_order_moment,
 // src/HDistribution.cpp:86-113), in the model's parameter names: p = (v0, theta, rho, kappa, sigma).
@@ -100,7 +105,8 @@ __host__ __device__ inline int32_t swift_grid_finish(const SwiftGridIn& q, uint3
 
 // The grid of chains in[0..n) on the GPU (one wavefront per chain; the scales m = 0..30 are tried
 // in parallel and the first that reaches the truncation precision is taken, as the reference's
-// loop takes it).
-hipError_t launch_swift_grid(const SwiftGridIn* in, SwiftGridOut* out, uint32_t n, hipStream_t st);
+// loop takes it), with each chain's extreme strikes from `strikes`.
+hipError_t launch_swift_grid(const SwiftGridIn* in, const double* strikes, SwiftGridOut* out, uint32_t n,
+                             hipStream_t st);
 
 }  // namespace hexo

@@ -1,7 +1,8 @@
 // swift_host.cpp -- host side of the SWIFT European pricer: validation, the dynamic grid (the
 // reference's SwiftParameters constructor, src/SWIFT.cpp:21-35) for hexo_swift_grid_for (a call's
-// grids are chosen on the GPU, swift_grid.hpp), packing of one call's chains into one device
-// region, and the C entry points hexo_swift_grid_for / hexo_swift_price.
+// grids are chosen on the GPU, swift_grid.hpp), the layout of one call's chains in the context's
+// pinned stages and device workspace, and the C entry points hexo_swift_grid_for /
+// hexo_swift_price.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,7 +43,8 @@ double int_error(const double p[5], double tau, unsigned m) {
 
 bool finite_pos(double x) { return std::isfinite(x) && x > 0.0; }
 
-int check_request(const hexo_swift_request& q, std::string& err) {
+// Everything but the strikes' values.
+int check_chain(const hexo_swift_request& q, std::string& err) {
     if (!(finite_pos(q.v0) && finite_pos(q.theta) && finite_pos(q.kappa) && finite_pos(q.sigma) &&
           std::isfinite(q.rho) && q.rho > -1.0 && q.rho < 1.0)) {
         err = "Heston parameters out of range (v0, theta, kappa, sigma > 0, |rho| < 1)";
@@ -56,11 +58,6 @@ int check_request(const hexo_swift_request& q, std::string& err) {
         err = "a chain needs at least one strike";
         return HEXO_ERR_INVALID;
     }
-    for (uint32_t i = 0; i < q.n_strikes; ++i)
-        if (!finite_pos(q.strikes[i])) {
-            err = "strikes must be positive";
-            return HEXO_ERR_INVALID;
-        }
     if (q.flags & ~HEXO_SWIFT_FIXED_GRID) {
         err = "unknown flags";
         return HEXO_ERR_INVALID;
@@ -70,6 +67,20 @@ int check_request(const hexo_swift_request& q, std::string& err) {
         return HEXO_ERR_INVALID;
     }
     return HEXO_OK;
+}
+
+int check_strikes(const hexo_swift_request& q, std::string& err) {
+    for (uint32_t i = 0; i < q.n_strikes; ++i)
+        if (!finite_pos(q.strikes[i])) {
+            err = "strikes must be positive";
+            return HEXO_ERR_INVALID;
+        }
+    return HEXO_OK;
+}
+
+int check_request(const hexo_swift_request& q, std::string& err) {
+    const int rc = check_chain(q, err);
+    return rc ? rc : check_strikes(q, err);
 }
 
 int check_grid(const hexo_swift_grid& g, std::string& err) {
@@ -94,7 +105,8 @@ int check_grid(const hexo_swift_grid& g, std::string& err) {
 // SwiftParameters constructor (src/SWIFT.cpp:21-35).
 // The extreme strikes and the rest of a chain's grid-choice inputs.
 hexo::SwiftGridIn grid_input(const hexo_swift_request& q) {
-    hexo::SwiftGridIn in{{q.v0, q.theta, q.rho, q.kappa, q.sigma}, q.T, q.r, q.S0, q.strikes[0], q.strikes[0], q.trunc};
+    hexo::SwiftGridIn in{{q.v0, q.theta, q.rho, q.kappa, q.sigma}, q.T, q.r, q.S0, q.strikes[0], q.strikes[0], q.trunc,
+                         0, q.n_strikes, 0};
     for (uint32_t i = 1; i < q.n_strikes; ++i) {
         in.kmin = std::min(in.kmin, q.strikes[i]);
         in.kmax = std::max(in.kmax, q.strikes[i]);
@@ -107,6 +119,7 @@ int grid_status(int32_t status, double J_wanted, std::string& err) {
         case hexo::kSwiftGridOk: return HEXO_OK;
         case hexo::kSwiftGridNoScale: err = "no wavelet scale reaches the truncation precision"; break;
         case hexo::kSwiftGridOverflow: err = "SWIFT index range overflows"; break;
+        case hexo::kSwiftGridBadStrike: err = "strikes must be positive"; break;
         default: err = "SWIFT grid needs J = " + std::to_string(J_wanted) + " (outside [1, HEXO_SWIFT_MAX_J])"; break;
     }
     return HEXO_ERR_INVALID;
@@ -124,20 +137,21 @@ int dynamic_grid(const hexo_swift_request& q, hexo_swift_grid& g, std::string& e
     return grid_status(hexo::swift_grid_finish(in, m, g, J), J, err);
 }
 
-// Grows the context's SWIFT device workspace to at least `bytes` (kept for the context's lifetime).
-int swift_workspace(hexo_ctx* ctx, size_t bytes) {
-    if (bytes <= ctx->swift_dev_bytes) return HEXO_OK;
-    // the previous workspace may still be read by work queued on the stream
+// Grows one of the context's SWIFT device buffers to at least `bytes` (kept for the context's
+// lifetime, so calls of a calibration loop allocate nothing).
+int grow_device(hexo_ctx* ctx, void*& buf, size_t& cap, size_t bytes) {
+    if (bytes <= cap) return HEXO_OK;
+    // the previous buffer may still be read by work queued on the stream
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ctx->set_error(HEXO_ERR_HIP, "stream sync failed");
-    if (ctx->swift_dev) (void)hipFree(ctx->swift_dev);
-    ctx->swift_dev = nullptr;
-    ctx->swift_dev_bytes = 0;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
     const size_t want = bytes + bytes / 2;
-    if (hipMalloc(&ctx->swift_dev, want) != hipSuccess) {
-        ctx->swift_dev = nullptr;
+    if (hipMalloc(&buf, want) != hipSuccess) {
+        buf = nullptr;
         return ctx->set_error(HEXO_ERR_NOMEM, "hipMalloc failed");
     }
-    ctx->swift_dev_bytes = want;
+    cap = want;
     return HEXO_OK;
 }
 
@@ -164,64 +178,60 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
     if (!ctx) return HEXO_ERR_INVALID;
     if (!reqs || !prices || n == 0) return ctx->set_error(HEXO_ERR_INVALID, "requests and prices must be non-NULL");
     std::string err;
-    std::vector<hexo_swift_grid> grids(n);
+    std::vector<uint64_t> strike_off(n);
     std::vector<uint32_t> dyn;  // chains whose grid is chosen here (not HEXO_SWIFT_FIXED_GRID)
+    uint64_t n_strk = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        const int rc = check_request(reqs[i], err);
-        if (rc) return ctx->set_error(rc, "swift request " + std::to_string(i) + ": " + err);
-        if (reqs[i].flags & HEXO_SWIFT_FIXED_GRID)
-            grids[i] = reqs[i].grid;
-        else
-            dyn.push_back(i);
+        if (int rc = check_chain(reqs[i], err))
+            return ctx->set_error(rc, "swift request " + std::to_string(i) + ": " + err);
+        strike_off[i] = n_strk;
+        n_strk += reqs[i].n_strikes;
+        if (!(reqs[i].flags & HEXO_SWIFT_FIXED_GRID)) dyn.push_back(i);
     }
     std::lock_guard<std::recursive_mutex> lock(ctx->mu);
     if (hipSetDevice(ctx->device) != hipSuccess) return ctx->set_error(HEXO_ERR_HIP, "hipSetDevice failed");
     auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    if (!dyn.empty()) {
-        // the dynamic grids of all chains in one launch (a wavefront per chain), through the
-        // call's pinned stage and device workspace, which the packing below then reuses
-        const size_t nd = dyn.size();
-        const size_t b_in = align(sizeof(hexo::SwiftGridIn) * nd), b_out = sizeof(hexo::SwiftGridOut) * nd;
-        if (!ctx->swift_host.reserve(b_in + b_out)) return ctx->set_error(HEXO_ERR_NOMEM, "hipHostMalloc failed");
-        if (int rc = swift_workspace(ctx, b_in + b_out)) return rc;
-        unsigned char* host = reinterpret_cast<unsigned char*>(ctx->swift_host.p);
-        unsigned char* dev = reinterpret_cast<unsigned char*>(ctx->swift_dev);
-        hexo::SwiftGridIn* gin = reinterpret_cast<hexo::SwiftGridIn*>(host);
-        for (size_t k = 0; k < nd; ++k) gin[k] = grid_input(reqs[dyn[k]]);
-        hipStream_t st = ctx->stream;
-        const bool ok =
-            hipMemcpyAsync(dev, host, sizeof(hexo::SwiftGridIn) * nd, hipMemcpyHostToDevice, st) == hipSuccess &&
-            hexo::launch_swift_grid(reinterpret_cast<const hexo::SwiftGridIn*>(dev),
-                                    reinterpret_cast<hexo::SwiftGridOut*>(dev + b_in), (uint32_t)nd, st) ==
-                hipSuccess &&
-            hipMemcpyAsync(host + b_in, dev + b_in, b_out, hipMemcpyDeviceToHost, st) == hipSuccess &&
-            hipStreamSynchronize(st) == hipSuccess;
-        if (!ok) {
-            (void)hipStreamSynchronize(st);
-            return ctx->set_error(HEXO_ERR_HIP, "SWIFT grid launch failed");
-        }
-        const hexo::SwiftGridOut* gout = reinterpret_cast<const hexo::SwiftGridOut*>(host + b_in);
-        for (size_t k = 0; k < nd; ++k) {
-            if (int rc = grid_status(gout[k].status, gout[k].J_wanted, err))
-                return ctx->set_error(rc, "swift request " + std::to_string(dyn[k]) + ": " + err);
-            grids[dyn[k]] = gout[k].g;
-        }
+    hipStream_t st = ctx->stream;
+    // ---- first phase: the grid kernel reads the strikes and grid inputs from the fine-grained
+    // pinned stage and writes the grids back into it (a wavefront per dynamic chain, which also
+    // finds the chain's extreme strikes and checks them). Stage layout:
+    // [strikes | grid inputs | grid outputs].
+    const size_t nd = dyn.size();
+    const size_t b_strk = align(8 * n_strk), b_gin = align(sizeof(hexo::SwiftGridIn) * nd),
+                 b_gout = sizeof(hexo::SwiftGridOut) * nd;
+    if (!ctx->swift_in_host.reserve(b_strk + b_gin + b_gout)) return ctx->set_error(HEXO_ERR_NOMEM, "hipHostMalloc failed");
+    unsigned char* in_host = reinterpret_cast<unsigned char*>(ctx->swift_in_host.p);
+    const double* h_strikes = reinterpret_cast<const double*>(in_host);
+    {
+        double* ks = reinterpret_cast<double*>(in_host);
+        for (uint32_t i = 0; i < n; ++i) std::memcpy(ks + strike_off[i], reqs[i].strikes, 8 * reqs[i].n_strikes);
     }
+    hexo::SwiftGridOut* gout = reinterpret_cast<hexo::SwiftGridOut*>(in_host + b_strk + b_gin);
+    bool ok = true;
+    if (nd) {
+        hexo::SwiftGridIn* gin = reinterpret_cast<hexo::SwiftGridIn*>(in_host + b_strk);
+        for (size_t k = 0; k < nd; ++k) {
+            const hexo_swift_request& q = reqs[dyn[k]];
+            gin[k] = hexo::SwiftGridIn{{q.v0, q.theta, q.rho, q.kappa, q.sigma}, q.T, q.r, q.S0, 0.0, 0.0, q.trunc,
+                                       strike_off[dyn[k]], q.n_strikes, 0};
+        }
+        ok = hexo::launch_swift_grid(gin, h_strikes, gout, (uint32_t)nd, st) == hipSuccess;
+    }
+    auto fail = [&](int code, const std::string& msg) {
+        // drain what was queued before the pinned stages are reused
+        (void)hipStreamSynchronize(st);
+        return ctx->set_error(code, msg);
+    };
+    if (!ok) return fail(HEXO_ERR_HIP, "SWIFT grid launch failed");
+    // while the GPU chooses the grids: the strikes of the fixed-grid chains (the grid kernel checks
+    // the others') and the records' grid-independent fields
     for (uint32_t i = 0; i < n; ++i)
-        if (int rc = check_grid(grids[i], err))
-            return ctx->set_error(rc, "swift request " + std::to_string(i) + ": " + err);
-    // host image of the device region: records, tables, FFT chain list, prefix sums, strikes;
-    // then the work arrays
-    std::vector<hexo::DevSwift> recs(n);
-    std::vector<hexo::SwiftTable> tables;
-    std::vector<uint32_t> fft_chains;
-    std::vector<uint64_t> ends(4 * (size_t)n);  // j, dens, dcoef, strike blocks
-    uint64_t n_strk = 0, n_pay = 0, n_dens = 0, n_pm = 0;
-    hexo::SwiftTotals tot{0, 0, 0, 0, 0, 0, 0};
+        if (reqs[i].flags & HEXO_SWIFT_FIXED_GRID)
+            if (int rc = check_strikes(reqs[i], err)) return fail(rc, "swift request " + std::to_string(i) + ": " + err);
     const uint32_t rows = grads ? 6u : 1u;
+    std::vector<hexo::DevSwift> recs(n);
     for (uint32_t i = 0; i < n; ++i) {
         const hexo_swift_request& q = reqs[i];
-        const hexo_swift_grid& g = grids[i];
         hexo::DevSwift& R = recs[i];
         std::memset(&R, 0, sizeof(R));
         R.p[0] = q.v0;
@@ -233,6 +243,36 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
         R.r = q.r;
         R.T = q.T;
         R.disc = std::exp(-q.r * q.T);
+        R.n_strikes = q.n_strikes;
+        R.rows = rows;
+        R.strike_off = strike_off[i];
+        R.out_off = strike_off[i];
+    }
+    if (nd && hipStreamSynchronize(st) != hipSuccess) return fail(HEXO_ERR_HIP, "SWIFT grid launch failed");
+    std::vector<hexo_swift_grid> grids(n);
+    {
+        size_t k = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (reqs[i].flags & HEXO_SWIFT_FIXED_GRID) {
+                grids[i] = reqs[i].grid;
+            } else {
+                if (int rc = grid_status(gout[k].status, gout[k].J_wanted, err))
+                    return fail(rc, "swift request " + std::to_string(i) + ": " + err);
+                grids[i] = gout[k++].g;
+            }
+            if (int rc = check_grid(grids[i], err)) return fail(rc, "swift request " + std::to_string(i) + ": " + err);
+        }
+    }
+    // ---- second phase: the grid-dependent fields, tables, FFT chain list and prefix sums, then
+    // the work arrays
+    std::vector<hexo::SwiftTable> tables;
+    std::vector<uint32_t> fft_chains;
+    std::vector<uint64_t> ends(4 * (size_t)n);  // j, dens, dcoef, strike blocks
+    uint64_t n_pay = 0, n_dens = 0, n_pm = 0;
+    hexo::SwiftTotals tot{0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = 0; i < n; ++i) {
+        const hexo_swift_grid& g = grids[i];
+        hexo::DevSwift& R = recs[i];
         R.exp2_m = (double)g.exp2_m;
         R.sqrt_exp2_m = g.sqrt_exp2_m;
         R.lower = g.lower;
@@ -240,8 +280,6 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
         R.k1 = g.k1;
         R.k2 = g.k2;
         R.J = g.J;
-        R.n_strikes = q.n_strikes;
-        R.rows = rows;
         R.log2_2J = 1u + (uint32_t)__builtin_ctz(g.J);
         // one twiddle table per distinct J
         size_t t = 0;
@@ -251,8 +289,6 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
             tot.tw += 4ull * g.J;
         }
         R.tw_off = tables[t].off;
-        R.strike_off = n_strk;
-        R.out_off = n_strk;
         R.pay_off = n_pay;
         R.coeff_off = n_pay;
         R.dens_off = n_dens;
@@ -266,12 +302,11 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
             tot.dens += nk;
             tot.dcoef += g.J;
         }
-        n_strk += q.n_strikes;
         n_pay += g.J;
         n_dens += nk;
         n_pm += (uint64_t)rows * g.J;
         tot.j += g.J;
-        tot.sblocks += (q.n_strikes + hexo::kSwiftStrikeBlock - 1) / hexo::kSwiftStrikeBlock;
+        tot.sblocks += (reqs[i].n_strikes + hexo::kSwiftStrikeBlock - 1) / hexo::kSwiftStrikeBlock;
         ends[i] = tot.j;
         ends[n + i] = tot.dens;
         ends[2 * (size_t)n + i] = tot.dcoef;
@@ -283,16 +318,14 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
     const uint32_t n_tab = (uint32_t)tables.size();
     const size_t b_recs = align(sizeof(hexo::DevSwift) * n), b_tabs = align(sizeof(hexo::SwiftTable) * n_tab),
                  b_tabend = align(8 * (size_t)n_tab), b_fft = align(4 * fft_chains.size()),
-                 b_ends = align(sizeof(uint64_t) * 4 * (size_t)n), b_strk = align(8 * n_strk),
-                 b_tw = align(16 * tot.tw), b_pay = align(16 * n_pay), b_dens = align(8 * n_dens),
-                 b_coeff = align(16 * n_pay), b_pm = align(16 * n_pm), b_prices = align(8 * n_strk),
-                 b_grads = grads ? align(40 * n_strk) : 0;
-    const size_t in_bytes = b_recs + b_tabs + b_tabend + b_fft + b_ends + b_strk;
-    const size_t total = in_bytes + b_tw + b_pay + b_dens + b_coeff + b_pm + b_prices + b_grads;
-    const size_t out_bytes = 8 * n_strk + (grads ? 40 * n_strk : 0);
-    if (!ctx->swift_host.reserve(std::max(in_bytes, out_bytes)))
-        return ctx->set_error(HEXO_ERR_NOMEM, "hipHostMalloc failed");
-    if (int rc = swift_workspace(ctx, total)) return rc;
+                 b_ends = align(sizeof(uint64_t) * 4 * (size_t)n), b_tw = align(16 * tot.tw),
+                 b_pay = align(16 * n_pay), b_dens = align(8 * n_dens), b_coeff = align(16 * n_pay),
+                 b_pm = align(16 * n_pm), b_prices = align(8 * n_strk), b_grads = grads ? align(40 * n_strk) : 0;
+    const size_t in_bytes = b_recs + b_tabs + b_tabend + b_fft + b_ends;
+    // device: [tables | strikes | work arrays]; pinned stage: [tables | prices | grads]
+    const size_t total = in_bytes + b_strk + b_tw + b_pay + b_dens + b_coeff + b_pm;
+    if (!ctx->swift_host.reserve(in_bytes + b_prices + b_grads)) return fail(HEXO_ERR_NOMEM, "hipHostMalloc failed");
+    if (int rc = grow_device(ctx, ctx->swift_dev, ctx->swift_dev_bytes, total)) return rc;
     unsigned char* host = reinterpret_cast<unsigned char*>(ctx->swift_host.p);
     std::memset(host, 0, in_bytes);
     size_t off = 0;
@@ -305,11 +338,6 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
     if (!fft_chains.empty()) std::memcpy(host + off, fft_chains.data(), 4 * fft_chains.size());
     off += b_fft;
     std::memcpy(host + off, ends.data(), sizeof(uint64_t) * 4 * (size_t)n);
-    off += b_ends;
-    {
-        double* ks = reinterpret_cast<double*>(host + off);
-        for (uint32_t i = 0; i < n; ++i) std::memcpy(ks + recs[i].strike_off, reqs[i].strikes, 8 * reqs[i].n_strikes);
-    }
     unsigned char* dev = reinterpret_cast<unsigned char*>(ctx->swift_dev);
     hexo::SwiftArgs a{};
     off = 0;
@@ -329,7 +357,8 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
     a.dcoef_end = d_ends + 2 * (size_t)n;
     a.sblk_end = d_ends + 3 * (size_t)n;
     off += b_ends;
-    a.strikes = reinterpret_cast<const double*>(dev + off);
+    double* d_strikes = reinterpret_cast<double*>(dev + off);
+    a.strikes = d_strikes;
     off += b_strk;
     a.tw = reinterpret_cast<hexo::Cplx*>(dev + off);
     off += b_tw;
@@ -340,26 +369,16 @@ extern "C" int hexo_swift_price(hexo_ctx* ctx, const hexo_swift_request* reqs, u
     a.coeff = reinterpret_cast<hexo::Cplx*>(dev + off);
     off += b_coeff;
     a.pm = reinterpret_cast<hexo::Cplx*>(dev + off);
-    off += b_pm;
-    a.prices = reinterpret_cast<double*>(dev + off);
-    off += b_prices;
-    a.grads = grads ? reinterpret_cast<double*>(dev + off) : nullptr;
-    hipStream_t st = ctx->stream;
-    // inputs from the pinned stage; results back into it (the same buffer: the upload has
-    // completed when the stream reaches the result copies)
-    const bool ok = hipMemcpyAsync(dev, host, in_bytes, hipMemcpyHostToDevice, st) == hipSuccess &&
-                    hexo::launch_swift(a, tot, st) == hipSuccess &&
-                    hipMemcpyAsync(host, a.prices, 8 * n_strk, hipMemcpyDeviceToHost, st) == hipSuccess &&
-                    (!grads || hipMemcpyAsync(host + 8 * n_strk, a.grads, 40 * n_strk, hipMemcpyDeviceToHost, st) ==
-                                   hipSuccess) &&
-                    hipStreamSynchronize(st) == hipSuccess;
-    if (!ok) {
-        // drain what was queued before the pinned stage is reused
-        (void)hipStreamSynchronize(st);
-        return ctx->set_error(HEXO_ERR_HIP, "SWIFT launch failed");
-    }
-    std::memcpy(prices, host, 8 * n_strk);
-    if (grads) std::memcpy(grads, host + 8 * n_strk, 40 * n_strk);
+    // the contraction writes the results straight into the pinned stage
+    double* h_prices = reinterpret_cast<double*>(host + in_bytes);
+    double* h_grads = reinterpret_cast<double*>(host + in_bytes + b_prices);
+    a.prices = h_prices;
+    a.grads = grads ? h_grads : nullptr;
+    ok = hexo::launch_swift_stage(host, dev, in_bytes, h_strikes, d_strikes, b_strk, st) == hipSuccess &&
+         hexo::launch_swift(a, tot, st) == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) return fail(HEXO_ERR_HIP, "SWIFT launch failed");
+    std::memcpy(prices, h_prices, 8 * n_strk);
+    if (grads) std::memcpy(grads, h_grads, 40 * n_strk);
     if (grids_used) std::memcpy(grids_used, grids.data(), sizeof(hexo_swift_grid) * n);
     return HEXO_OK;
 }
