@@ -7,7 +7,7 @@
 //   hexo -t rng | -t simulation                                  self-tests on the GPU
 //   hexo -h
 // The reference downloads quotes and chains over HTTPS and keeps calibrated parameters in
-// sqlite (src/Main.cpp:64-86). This build reads them offline from $HEXO_DATA_DIR (default ./data):
+// sqlite (src/Main.cpp:64-86). This CLI reads them offline from $HEXO_DATA_DIR (default ./data):
 // the Tradier JSON responses saved under <underlying>/ (market_data.hpp), or a compact chain file
 // <underlying>.chains; parameters live in HParams.db there (the reference's table) or, without
 // libsqlite3, in <underlying>.params. As in the reference, -p uses stored parameters (a `params`
