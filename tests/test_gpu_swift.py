@@ -111,6 +111,45 @@ def test_swift_batch_equals_single(ctx):
         assert g is None
 
 
+def test_swift_grid_kernel_strikes_beyond_one_wavefront(ctx, oracle):
+    """The grid kernel takes each chain's extreme strikes with its 64 lanes striding over the
+    strikes: chains of 1, 63, 64, 65 and 300 unsorted strikes get the host's grid, and the oracle's
+    prices."""
+    say("gpu swift grid kernel, many strikes")
+    rng = np.random.default_rng(23)
+    chains = [H.SwiftChain(0.05, 0.05, -0.5, 1.5, 0.4, S0=100.0, T=T, r=0.02,
+                           strikes=list(rng.permutation(np.linspace(40.0, 220.0, k))))
+              for k, T in ((1, 0.3), (63, 0.7), (64, 1.0), (65, 2.0), (300, 4.0))]
+    res = ctx.swift_price(chains)
+    for ch, (p, _, used) in zip(chains, res):
+        host = H.swift_grid(ch)
+        assert [used[i] for i in (0, 1, 5, 6, 7)] == [host[i] for i in (0, 1, 5, 6, 7)]
+        assert np.allclose(used, host, rtol=1e-10, atol=0)
+        op, _ = oracle.swift_price([ch.v0, ch.theta, ch.rho, ch.kappa, ch.sigma], ch.S0, ch.r, ch.T,
+                                   ch.strikes, trunc=ch.trunc)
+        np.testing.assert_allclose(p, op, rtol=1e-10, atol=1e-11 * ch.S0)
+
+
+def test_swift_rejects_bad_strikes_in_any_chain(ctx):
+    """A non-positive or non-finite strike is refused whether the chain's grid is chosen on the
+    GPU (checked by the grid kernel) or given (checked on the host), wherever it sits."""
+    ok = H.SwiftChain(0.04, 0.04, -0.7, 2.0, 0.3, S0=100.0, T=1.0, strikes=list(np.linspace(70, 130, 100)))
+    fixed = H.SwiftChain(0.04, 0.04, -0.7, 2.0, 0.3, S0=100.0, T=1.0, strikes=[90.0, 110.0],
+                         grid=list(H.swift_grid(ok)))
+    for bad in (0.0, -5.0, float("nan"), float("inf")):
+        for pos in (0, 77, 99):
+            k = list(ok.strikes)
+            k[pos] = bad
+            with pytest.raises(H.HexoError):
+                ctx.swift_price([ok, H.SwiftChain(0.04, 0.04, -0.7, 2.0, 0.3, S0=100.0, T=1.0, strikes=k)])
+        with pytest.raises(H.HexoError):
+            ctx.swift_price([ok, H.SwiftChain(0.04, 0.04, -0.7, 2.0, 0.3, S0=100.0, T=1.0, strikes=[90.0, bad],
+                                              grid=list(fixed.grid))])
+    # and the context still prices afterwards
+    p, _, _ = ctx.swift_price([ok, fixed])[1]
+    assert np.all(np.isfinite(p))
+
+
 def test_swift_rejects_malformed(ctx):
     bad = H.SwiftChain(0.04, 0.04, -0.7, 2.0, 0.3, S0=100.0, T=1.0, strikes=[100.0], grid=[5, 32, 5.65, -1, 1, 10, 5, 128])
     with pytest.raises(H.HexoError):
