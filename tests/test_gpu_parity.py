@@ -61,6 +61,23 @@ def test_normals_fp64_and_fp32(ctx, oracle):
     np.testing.assert_allclose(g32, c32, rtol=2e-7, atol=2e-7)
 
 
+def test_fp32_normals_whole_tail_region(ctx, oracle):
+    """Every word whose folded value j = min(k, ~k) is below 2^21 + 2^16 (the whole tail-polynomial
+    region, both ends, and the central polynomial just past the switch): the GPU's FP32 normals
+    against the oracle's restatement and FP64 PPND16, and exactly antisymmetric on the GPU."""
+    say("gpu fp32 normals, whole tail region")
+    lo = np.arange(0, 2**21 + 2**16, dtype=np.uint64)
+    w = np.concatenate([lo, 2**32 - 1 - lo]).astype(np.uint32)
+    g32 = ctx.normals(w, H.FP32)
+    c32 = oracle.normals_f32(w).astype(np.float64)
+    c64 = oracle.normals_f64(w)
+    np.testing.assert_allclose(g32, c32, rtol=2e-7, atol=2e-7)
+    np.testing.assert_allclose(g32, c64, rtol=4e-7, atol=1e-6)
+    n = len(lo)
+    assert np.array_equal(g32[n:], -g32[:n])
+    assert np.all(np.diff(g32[:n]) >= -2e-6)  # monotone up to the switch's step
+
+
 def small_requests():
     base = dict(v0=0.04, theta=0.04, rho=-0.7, kappa=2.0, sigma=0.3, S0=100.0)
     return {
